@@ -1,0 +1,198 @@
+"""Throughput of the MI355X MD-step solver on one GPU.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Workload: the 80,000-lipid vesicle (240,000 beads) from the liposome generator (seed 1234,
+areal density 3.45, box 400^3, dt 0.02, T 3, gamma 1, tension 0 so no box moves), built
+in memory with the product's generator (byte-identical to the reference generator).  A
+step is one full MD step: Verlet first half + cell key, radix sort, reorder + cell table,
+pair force, CHAIN forces + Langevin + second half-kick.  Inputs are resident in HBM
+before timing starts; no measurement or file output inside the timed loop.
+
+Prints one JSON line: particle-steps/s, the pair kernel's roofline record (algorithmic
+bytes per launch over its HIP-event-timed average duration) and the CPU baseline (the
+reference MD binary built from source in oracle/_ref, or the FP64 oracle port when it is
+absent), timed on this host on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s at 1 GPU, plain vs batched FP32, and % of HBM roofline"
+HBM_PEAK_GBS = 8000.0
+VALU_FP32_PEAK_TFLOPS = 157.3
+SEED, NLIPIDS, DENSITY = 1234, 80000, 3.45
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def stencil_stats(keys: np.ndarray, nc) -> tuple[int, float]:
+    """Occupied cells and mean 27-stencil candidates per bead."""
+    nx, ny, nz = (int(x) for x in nc)
+    uk, cnt = np.unique(keys, return_counts=True)
+    occ = dict(zip(uk.tolist(), cnt.tolist()))
+    x, y, z = uk % nx, (uk // nx) % ny, uk // (nx * ny)
+    cand = np.zeros(len(uk))
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                kk = ((x + dx) % nx) + ((y + dy) % ny) * nx + ((z + dz) % nz) * nx * ny
+                cand += np.array([occ.get(int(k), 0) for k in kk])
+    return len(uk), float((cand * cnt).sum() / cnt.sum())
+
+
+def cpu_baseline(n_beads: int) -> dict:
+    """Reference MD binary (oracle/_ref/MD, compiled from the reference sources) on this
+    host: (wall of S steps - wall of 0 steps) / S with store and measure disabled."""
+    import softmold_amd as sm
+
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = int(os.environ.get("OMP_NUM_THREADS", cores))
+    md = os.path.join(ROOT, "oracle", "_ref", "MD")
+    work = tempfile.mkdtemp(prefix="smcpu_")
+    try:
+        m = sm.Mpd.liposome(SEED, NLIPIDS, DENSITY)
+        m["storeInterval"] = 1.0e6
+        m["measureInterval"] = 1.0e6
+        env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+        if os.access(md, os.X_OK):
+            def run(steps):
+                m["finalTime"] = steps * 0.02
+                m["initialTime"] = 0.0
+                m.write(os.path.join(work, "cpu"))
+                t0 = time.perf_counter()
+                subprocess.run([md, "cpu"], cwd=work, env=env, check=True, capture_output=True)
+                return time.perf_counter() - t0
+            t_zero = run(0)
+            t_probe = run(10) - t_zero
+            steps = int(max(20, min(2000, 15.0 / max(t_probe / 10, 1e-4))))
+            dt = run(steps) - t_zero
+            return {"value": n_beads * steps / dt, "unit": "particle-steps/s", "cores": cores,
+                    "kind": "reference",
+                    "sample": f"MD.cpp built from source, {steps} steps of the same vesicle "
+                              f"(wall of {steps} steps minus wall of 0 steps), OMP_NUM_THREADS={cores}"}
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        O.set_threads(cores)
+        d = m.system()
+        s = O.System(d["pos"], d["type"], d["vel"], d["box"], d["fconst"], d["uconst"], d["n_types"],
+                     d["chain"], d["chain_const"], cutoff=d["cutoff"], dt=d["dt"], gamma=d["gamma"],
+                     temp=d["temperature"], seed=d["seed"])
+        mdo = O.MD(s, rng_mode=1)
+        mdo.init_forces()
+        t0 = time.perf_counter()
+        mdo.run(0, 9)
+        probe = (time.perf_counter() - t0) / 10
+        steps = int(max(10, min(1000, 15.0 / max(probe, 1e-4))))
+        t0 = time.perf_counter()
+        mdo.run(10, 10 + steps - 1)
+        dt = time.perf_counter() - t0
+        return {"value": n_beads * steps / dt, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+                "sample": f"FP64 oracle restatement of MD.cpp, {steps} steps of the same vesicle"}
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(replicas would shard one subset per GPU; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import softmold_amd as sm
+
+    assert torch.cuda.is_available(), "bench.py needs the MI355X"
+    torch.cuda.set_device(0)
+    d = sm.Mpd.liposome(SEED, NLIPIDS, DENSITY).system()
+    n = len(d["type"])
+    s = sm.Solver.from_system(d)
+    keys = s.cell_keys()
+    nc = (d["box"] / d["cutoff"]).astype(int)
+    occ, cand = stencil_stats(keys, nc)
+    _, npairs = s.pair_count()
+    s.init_forces()
+
+    # warmup, then the timed loop bracketed by device synchronisation
+    s.run(0, args.warmup)
+    s.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.run(args.warmup, args.steps)
+    s.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    value = n * args.steps / wall
+
+    # per-kernel averages from HIP events on the solver's stream, same number of steps
+    s.profile(True)
+    s.run(args.warmup + args.steps, args.steps)
+    prof = s.profile_read()
+    s.profile(False)
+    pair_ms = prof["pair_force"][0]
+    for k, (avg, cnt) in prof.items():
+        if cnt:
+            log(f"  {k:26s} {avg * 1e3:9.2f} us x {cnt}")
+    algo_bytes = 28.0 * n + 8.0 * occ      # p16 read + f12 write per bead, 8 B per occupied cell
+    achieved = algo_bytes / (pair_ms * 1e-3) / 1e9
+    accepted = 2.0 * npairs / n
+    flops = n * (9.0 * cand + 14.0 * accepted)
+    tflops = flops / (pair_ms * 1e-3) / 1e12
+    log(f"N={n} occupied cells={occ} candidates/bead={cand:.1f} neighbours/bead={accepted:.1f}")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair arithmetic, f64 positions/energies",
+        "data": "synthetic (liposome generator output, seed 1234)",
+        "config": {"workload": "plain solver, vesicle 80000 lipids = 240000 beads, box 400^3, "
+                               "dt 0.02, T 3, gamma 1, tension 0",
+                   "n_beads": n, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "pair_force", "kernel_avg_us": pair_ms * 1e3,
+                     "algorithmic_bytes_per_launch": algo_bytes,
+                     "valu_tflops": tflops, "valu_frac_fp32": tflops / VALU_FP32_PEAK_TFLOPS},
+    }
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(n)
+        except Exception as e:  # the GPU number stands on its own
+            log("cpu baseline failed:", e)
+            out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

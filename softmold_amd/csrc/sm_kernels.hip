@@ -1,0 +1,808 @@
+// gfx950 kernels of the MD step: integrate + cell key, reorder + dense cell table, the
+// LDS-staged Laradji-Revalee pair force, CHAIN bonds/bends fused with the Langevin force
+// and the second half-kick, FP64 energy / dU / neighbour gathers, deterministic FP64
+// reductions, and the device-side Monte-Carlo box move.
+#include "sm_internal.hpp"
+
+namespace smdev {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------------------
+// counter RNG: Philox4x32-10 (Salmon et al., SC'11); counter = (particle, step lo, step hi,
+// tag), key = (seed lo, seed hi).
+// ---------------------------------------------------------------------------------------
+__device__ inline uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    return make_uint4(c0, c1, c2, c3);
+}
+
+// psi in (-1,1) = ((x>>8) 2^-23 - 1) + 2^-24; every step is exact in binary32.
+__device__ inline float psi32(uint32_t x)
+{
+    return ((float)(x >> 8) * (1.0f / 8388608.0f) - 1.0f) + (1.0f / 16777216.0f);
+}
+
+__device__ inline double psi64(uint32_t x)
+{
+    return ((double)(x >> 8) * (1.0 / 8388608.0) - 1.0) + (1.0 / 16777216.0);
+}
+
+// Cell index along one axis: int() truncation of the FP64 quotient, then clamped into
+// [0, n-1] (the reference's one-cell correction at the top edge, cellOpt.h:532-539).
+__device__ inline int cell_axis(double p, double cs, int n)
+{
+    int c = (int)(p / cs);
+    c = c >= n ? n - 1 : c;
+    return c < 0 ? 0 : c;
+}
+
+__device__ inline bool in_box(double p, double L) { return p >= 0.0 && p <= L; }
+
+// ---------------------------------------------------------------------------------------
+// Verlet::first (verlet.h:289-356) + cell key (cellOpt.h:530-556)
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_first(int n, Bead *__restrict__ beads, float4 *__restrict__ vel,
+                                                  const float4 *__restrict__ acc, DevBox *__restrict__ box,
+                                                  uint32_t *__restrict__ key, uint32_t *__restrict__ idx,
+                                                  float halfdt, double dt, int integrate)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double L0 = box->L[0], L1 = box->L[1], L2 = box->L[2];
+    Bead b = beads[i];
+    if (integrate && b.type != 0) {
+        float4 v = vel[i];
+        const float4 a = acc[i];
+        v.x += a.x * halfdt;
+        v.y += a.y * halfdt;
+        v.z += a.z * halfdt;
+        b.x += (double)v.x * dt;
+        b.y += (double)v.y * dt;
+        b.z += (double)v.z * dt;
+        if (b.x > L0) b.x -= L0;
+        if (b.x < 0) b.x += L0;
+        if (b.y > L1) b.y -= L1;
+        if (b.y < 0) b.y += L1;
+        if (b.z > L2) b.z -= L2;
+        if (b.z < 0) b.z += L2;
+        vel[i] = v;
+        beads[i] = b;
+    }
+    const int nx = box->nc[0], ny = box->nc[1], nz = box->nc[2];
+    if (!(in_box(b.x, L0) && in_box(b.y, L1) && in_box(b.z, L2))) atomicOr(&box->flags, FLAG_BAD_POSITION);
+    const int cx = cell_axis(b.x, box->cs[0], nx);
+    const int cy = cell_axis(b.y, box->cs[1], ny);
+    const int cz = cell_axis(b.z, box->cs[2], nz);
+    key[i] = (uint32_t)(cx + cy * nx + cz * nx * ny);
+    idx[i] = (uint32_t)i;
+}
+
+// ---------------------------------------------------------------------------------------
+// reorder into cell order; cell runs written into the dense (never cleared) table
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_reorder(int n, const uint32_t *__restrict__ skey,
+                                                    const uint32_t *__restrict__ sidx, const Bead *__restrict__ beads,
+                                                    Bead *__restrict__ sbeads, int *__restrict__ cstart,
+                                                    int *__restrict__ cend)
+{
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t k = skey[s];
+    sbeads[s] = beads[sidx[s]];
+    if (s == 0 || skey[s - 1] != k) cstart[k] = s;
+    if (s == n - 1 || skey[s + 1] != k) cend[k] = s + 1;
+}
+
+__device__ inline int wave_incl_scan(int v, int lane)
+{
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int t = __shfl_up(v, off);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+__device__ inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; map them so each XCD gets one
+// contiguous range of the sorted particles (shared neighbour cells stay in one L2).
+__device__ inline int xcd_block(int b, int nblocks)
+{
+    const int x = b & 7, q = nblocks >> 3, r = nblocks & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
+// ---------------------------------------------------------------------------------------
+// Pair force.  One wave64 per chunk of `chunk` sorted slots; it owns the cells whose run
+// starts in the chunk.  Per cell: lanes 0..26 look up the 27 stencil cells in the dense
+// table, a wave scan lays their particles out in LDS (FP64, periodic image applied),
+// then tpp = 64/H lanes serve each of the H home particles and split the candidates;
+// partial forces are summed across the tpp lanes through LDS in lane order.
+// Position differences and the r^2 < rc^2 test are FP64 (no contraction, as MD.h:798-810
+// evaluates them); the force law is FP32 (MD.h:817-828; include/cuda/potentials.h:10-31).
+// ---------------------------------------------------------------------------------------
+constexpr int kPairWaves = 4;
+constexpr int kMaxC = 256;
+
+__global__ __launch_bounds__(kPairWaves * kWave) void k_pair_force(
+    int n, int nT, int chunk, int nblocks, double cut2, const Bead *__restrict__ sb,
+    const uint32_t *__restrict__ skey, const uint32_t *__restrict__ sidx, const int *__restrict__ cstart,
+    const int *__restrict__ cend, const DevBox *__restrict__ box, const float *__restrict__ fconst,
+    float4 *__restrict__ fout)
+{
+    __shared__ float sC[6 * kMaxTypes * kMaxTypes];
+    __shared__ double sX[kPairWaves][kMaxC];
+    __shared__ double sY[kPairWaves][kMaxC];
+    __shared__ double sZ[kPairWaves][kMaxC];
+    __shared__ int sT[kPairWaves][kMaxC];
+    __shared__ float red[kPairWaves][3][kWave];
+
+    for (int t = threadIdx.x; t < 6 * nT * nT; t += blockDim.x) sC[t] = fconst[t];
+    __syncthreads();
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const long long a0 = ((long long)xcd_block(blockIdx.x, nblocks) * kPairWaves + wave) * chunk;
+    if (a0 >= n) return;
+    const int a = (int)a0;
+    const int aend = min(a + chunk, n);
+    const int nx = box->nc[0], ny = box->nc[1], nz = box->nc[2];
+    const double Lx = box->L[0], Ly = box->L[1], Lz = box->L[2];
+    double *X = sX[wave], *Y = sY[wave], *Z = sZ[wave];
+    int *T = sT[wave];
+
+    for (int base = a; base < aend; base += kWave) {
+        const int s = base + lane;
+        bool st = false;
+        if (s < aend) st = (s == 0) || (skey[s - 1] != skey[s]);
+        unsigned long long starts = __ballot(st);
+        while (starts) {
+            const int s0 = base + __ffsll((long long)starts) - 1;
+            starts &= starts - 1;
+            const uint32_t k = skey[s0];
+            const int nh = cend[k] - s0;
+            const int kx = (int)(k % (uint32_t)nx);
+            const int ky = (int)((k / (uint32_t)nx) % (uint32_t)ny);
+            const int kz = (int)(k / (uint32_t)(nx * ny));
+            int nst = 0, ncnt = 0;
+            double shx = 0.0, shy = 0.0, shz = 0.0;
+            if (lane < 27) {
+                int X0 = kx + lane % 3 - 1, Y0 = ky + (lane / 3) % 3 - 1, Z0 = kz + lane / 9 - 1;
+                if (X0 < 0) { X0 += nx; shx = -Lx; } else if (X0 >= nx) { X0 -= nx; shx = Lx; }
+                if (Y0 < 0) { Y0 += ny; shy = -Ly; } else if (Y0 >= ny) { Y0 -= ny; shy = Ly; }
+                if (Z0 < 0) { Z0 += nz; shz = -Lz; } else if (Z0 >= nz) { Z0 -= nz; shz = Lz; }
+                const uint32_t kk = (uint32_t)(X0 + Y0 * nx + Z0 * nx * ny);
+                const int c0 = cstart[kk];
+                if ((unsigned)c0 < (unsigned)n && skey[c0] == kk) {
+                    nst = c0;
+                    ncnt = cend[kk] - c0;
+                }
+            }
+            const int incl = wave_incl_scan(ncnt, lane);
+            const int excl = incl - ncnt;
+            const int total = __shfl(incl, 26);
+            const int selfOff = __shfl(excl, 13);
+
+            for (int h0 = 0; h0 < nh; h0 += kWave) {
+                const int H = min(kWave, nh - h0);
+                const int tpp = kWave / H;
+                const int hi = lane / tpp;
+                const int sub = lane - hi * tpp;
+                const bool act = hi < H;
+                double hx = 0.0, hy = 0.0, hz = 0.0;
+                int ht = 0;
+                if (act) {
+                    const Bead hb = sb[s0 + h0 + hi];
+                    hx = hb.x; hy = hb.y; hz = hb.z; ht = hb.type;
+                }
+                const int self = selfOff + h0 + hi;
+                const float *Crow = sC + 6 * ht * nT;
+                float fx = 0.f, fy = 0.f, fz = 0.f;
+                for (int cb = 0; cb < total; cb += kMaxC) {
+                    const int ce = min(total, cb + kMaxC);
+                    for (int c = 0; c < 27; c++) {
+                        const int ccnt = __shfl(ncnt, c);
+                        if (ccnt == 0) continue;
+                        const int coff = __shfl(excl, c);
+                        const int lo = max(coff, cb), hi2 = min(coff + ccnt, ce);
+                        if (lo >= hi2) continue;
+                        const int cst = __shfl(nst, c);
+                        const double sx = __shfl(shx, c), sy = __shfl(shy, c), sz = __shfl(shz, c);
+                        for (int g = lo + lane; g < hi2; g += kWave) {
+                            const Bead q = sb[cst + (g - coff)];
+                            X[g - cb] = q.x + sx;
+                            Y[g - cb] = q.y + sy;
+                            Z[g - cb] = q.z + sz;
+                            T[g - cb] = q.type;
+                        }
+                    }
+                    wave_lds_sync();
+                    if (act) {
+                        for (int j = cb + sub; j < ce; j += tpp) {
+                            const int jj = j - cb;
+                            const double dxd = hx - X[jj];
+                            const double dyd = hy - Y[jj];
+                            const double dzd = hz - Z[jj];
+                            const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(dxd, dxd), __dmul_rn(dyd, dyd)),
+                                                        __dmul_rn(dzd, dzd));
+                            if (r2 < cut2 && j != self) {
+                                const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
+                                const float r = sqrtf((float)r2);
+                                const float *C = Crow + 6 * T[jj];
+                                const int off = (r >= C[0]) ? 3 : 0;
+                                const float m = C[off] - r;
+                                const float mag = ((C[off + 1] - C[off + 2] * m) * m) * __builtin_amdgcn_rcpf(r);
+                                fx += dx * mag;
+                                fy += dy * mag;
+                                fz += dz * mag;
+                            }
+                        }
+                    }
+                    wave_lds_sync();
+                }
+                red[wave][0][lane] = fx;
+                red[wave][1][lane] = fy;
+                red[wave][2][lane] = fz;
+                wave_lds_sync();
+                if (act && sub == 0) {
+                    float ax = 0.f, ay = 0.f, az = 0.f;
+                    for (int t = 0; t < tpp; t++) {
+                        ax += red[wave][0][lane + t];
+                        ay += red[wave][1][lane + t];
+                        az += red[wave][2][lane + t];
+                    }
+                    fout[sidx[s0 + h0 + hi]] = make_float4(ax, ay, az, 0.f);
+                }
+                wave_lds_sync();
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// CHAIN bonded forces by bead role (bond: harmonicF MD.h:384-403; bend: bendF
+// MD.h:683-723; chain walk system.h:1782-1866), fused with the Langevin force
+// (langevin.h:233-331) and Verlet::second (verlet.h:464-477).
+// ---------------------------------------------------------------------------------------
+__device__ inline float3 min_image(const Bead &p, const Bead &q, double L0, double L1, double L2)
+{
+    double dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    if (dx > L0 / 2.0) dx -= L0;
+    if (dx < -L0 / 2.0) dx += L0;
+    if (dy > L1 / 2.0) dy -= L1;
+    if (dy < -L1 / 2.0) dy += L1;
+    if (dz > L2 / 2.0) dz -= L2;
+    if (dz < -L2 / 2.0) dz += L2;
+    return make_float3((float)dx, (float)dy, (float)dz);
+}
+
+// harmonic force on the first particle of the bond (a1 += d * mag)
+__device__ inline float3 bond_f(float3 d, float l0, float kb)
+{
+    const float dr = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+    const float mag = -(dr - l0) * kb / dr;
+    return make_float3(d.x * mag, d.y * mag, d.z * mag);
+}
+
+// bend: returns the force on the particle with role 0, 1 or 2 of (p1, p2, p3)
+__device__ inline float3 bend_f(float3 da, float3 db, float c0, float kt, int role)
+{
+    const float dra = sqrtf(da.x * da.x + da.y * da.y + da.z * da.z);
+    const float drb = sqrtf(db.x * db.x + db.y * db.y + db.z * db.z);
+    da.x /= dra; da.y /= dra; da.z /= dra;
+    db.x /= drb; db.y /= drb; db.z /= drb;
+    const float cost = da.x * db.x + da.y * db.y + da.z * db.z;
+    const float mag = (c0 - cost) * kt;
+    const float ta = mag / dra, tb = mag / drb;
+    const float3 A = make_float3(ta * (db.x - da.x * cost), ta * (db.y - da.y * cost), ta * (db.z - da.z * cost));
+    const float3 B = make_float3(tb * (da.x - db.x * cost), tb * (da.y - db.y * cost), tb * (da.z - db.z * cost));
+    if (role == 0) return A;
+    if (role == 1) return make_float3(B.x - A.x, B.y - A.y, B.z - A.z);
+    return make_float3(-B.x, -B.y, -B.z);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bonded_kick(int n, const Bead *__restrict__ beads,
+                                                        const int2 *__restrict__ chain,
+                                                        const float4 *__restrict__ cconst,
+                                                        const float4 *__restrict__ fpair, float4 *__restrict__ vel,
+                                                        float4 *__restrict__ acc, const DevBox *__restrict__ box,
+                                                        unsigned long long seed, long long step, float gamma,
+                                                        float sigma, float halfdt, int mode,
+                                                        float4 *__restrict__ fchain_out)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double L0 = box->L[0], L1 = box->L[1], L2 = box->L[2];
+    float3 fc = make_float3(0.f, 0.f, 0.f);
+    const int2 ci = chain[i];
+    if (ci.x >= 0) {
+        const int start = ci.x, len = ci.y & 0xFFFF, cid = ci.y >> 16;
+        const float4 K = cconst[cid];
+        const int j = i - start;
+        const Bead p0 = beads[i];
+        Bead pm1, pm2, pp1, pp2;
+        if (j >= 1) pm1 = beads[i - 1];
+        if (j >= 2) pm2 = beads[i - 2];
+        if (j <= len - 2) pp1 = beads[i + 1];
+        if (j <= len - 3) pp2 = beads[i + 2];
+        if (j >= 1) {  // bond (i-1, i): this bead is a2
+            const float3 f = bond_f(min_image(pm1, p0, L0, L1, L2), K.x, K.y);
+            fc.x -= f.x; fc.y -= f.y; fc.z -= f.z;
+        }
+        if (j <= len - 2) {  // bond (i, i+1): this bead is a1
+            const float3 f = bond_f(min_image(p0, pp1, L0, L1, L2), K.x, K.y);
+            fc.x += f.x; fc.y += f.y; fc.z += f.z;
+        }
+        if (j <= len - 3) {  // bend (i, i+1, i+2), role 0
+            const float3 f = bend_f(min_image(p0, pp1, L0, L1, L2), min_image(pp1, pp2, L0, L1, L2), K.z, K.w, 0);
+            fc.x += f.x; fc.y += f.y; fc.z += f.z;
+        }
+        if (j >= 1 && j <= len - 2) {  // bend (i-1, i, i+1), role 1
+            const float3 f = bend_f(min_image(pm1, p0, L0, L1, L2), min_image(p0, pp1, L0, L1, L2), K.z, K.w, 1);
+            fc.x += f.x; fc.y += f.y; fc.z += f.z;
+        }
+        if (j >= 2) {  // bend (i-2, i-1, i), role 2
+            const float3 f = bend_f(min_image(pm2, pm1, L0, L1, L2), min_image(pm1, p0, L0, L1, L2), K.z, K.w, 2);
+            fc.x += f.x; fc.y += f.y; fc.z += f.z;
+        }
+    }
+    if (mode == KICK_NONE) {
+        fchain_out[i] = make_float4(fc.x, fc.y, fc.z, 0.f);
+        return;
+    }
+    const uint4 o = philox((uint32_t)i, (uint32_t)step, (uint32_t)((unsigned long long)step >> 32),
+                           mode == KICK_INIT ? TAG_LANGEVIN_INIT : TAG_LANGEVIN, (uint32_t)seed,
+                           (uint32_t)(seed >> 32));
+    float4 v = vel[i];
+    const float4 fp = fpair[i];
+    float4 a;
+    a.x = (-gamma * v.x + sigma * psi32(o.x));
+    a.y = (-gamma * v.y + sigma * psi32(o.y));
+    a.z = (-gamma * v.z + sigma * psi32(o.z));
+    a.x += fp.x; a.y += fp.y; a.z += fp.z;
+    a.x += fc.x; a.y += fc.y; a.z += fc.z;
+    a.w = 0.f;
+    acc[i] = a;
+    if (mode == KICK_STEP && beads[i].type != 0) {
+        v.x += a.x * halfdt;
+        v.y += a.y * halfdt;
+        v.z += a.z * halfdt;
+        vel[i] = v;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_second(int n, const Bead *__restrict__ beads,
+                                                   const float4 *__restrict__ acc, float4 *__restrict__ vel,
+                                                   float halfdt)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    if (beads[i].type == 0) return;
+    float4 v = vel[i];
+    const float4 a = acc[i];
+    v.x += a.x * halfdt;
+    v.y += a.y * halfdt;
+    v.z += a.z * halfdt;
+    vel[i] = v;
+}
+
+// ---------------------------------------------------------------------------------------
+// FP64 reductions: fixed-shape trees, so the result is independent of scheduling.
+// ---------------------------------------------------------------------------------------
+__device__ inline double block_sum(double v, double *lds)
+{
+    lds[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
+        __syncthreads();
+    }
+    const double r = lds[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ inline double reduce_partials(const double *p, int count, double *lds)
+{
+    double s = 0.0;
+    for (int i = threadIdx.x; i < count; i += kBlock) s += p[i];
+    return block_sum(s, lds);
+}
+
+__global__ __launch_bounds__(kBlock) void k_reduce(const double *__restrict__ partial, int count,
+                                                   double *__restrict__ out)
+{
+    __shared__ double lds[kBlock];
+    const double s = reduce_partials(partial, count, lds);
+    if (threadIdx.x == 0) *out = s;
+}
+
+// Potential<double> (MD.h:895-930)
+__device__ inline double pair_u(double r2, const double *C)
+{
+    const double dr = sqrt(r2);
+    double pot;
+    if (dr <= C[0]) {
+        pot = C[0] - dr;
+        pot = C[1] * pot * pot + C[2];
+    } else {
+        pot = C[3] - dr;
+        pot = pot * pot * (C[4] - pot * C[5]);
+    }
+    return pot;
+}
+
+__device__ inline double sq3(double x, double y, double z)
+{
+    return __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z));
+}
+
+// Forward half shell of the reference (cellOpt.h:715) as a 27-bit mask over stencil index
+// (dx+1) + 3(dy+1) + 9(dz+1); the pair (home cell, stencil cell) is owned by the home
+// cell iff the offset is in the mask.
+__device__ constexpr uint32_t forward_mask()
+{
+    const int nb[13][3] = {{1, 0, 0}, {0, 1, 0}, {1, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1},
+                           {-1, 0, 1}, {0, -1, 1}, {-1, 1, 1}, {1, -1, 1}, {-1, -1, 1}, {1, -1, 0}};
+    uint32_t m = 0;
+    for (int j = 0; j < 13; j++) m |= 1u << ((nb[j][0] + 1) + 3 * (nb[j][1] + 1) + 9 * (nb[j][2] + 1));
+    return m;
+}
+
+// One thread per sorted particle, FP64 throughout, differences taken in the direction the
+// reference evaluates each pair (owner cell's particle minus the imaged neighbour):
+//   GATHER_U     sum_j U(r_ij)/2                                 (computePotential)
+//   GATHER_DU    sum_j [U(d_ij) - U(scale o d_ij)]/2 over the stencil (computeDPotential)
+//   GATHER_COUNT neighbours with r^2 < rc^2                        (FORCE_COUNT)
+//   GATHER_LIST  pairs (i<j) with r^2 < rc^2
+__global__ __launch_bounds__(kBlock) void k_gather(int mode, int n, int nT, double cut2,
+                                                   const Bead *__restrict__ sb, const uint32_t *__restrict__ skey,
+                                                   const uint32_t *__restrict__ sidx, const int *__restrict__ cstart,
+                                                   const int *__restrict__ cend, const DevBox *__restrict__ box,
+                                                   const double *__restrict__ uconst, double *__restrict__ partial,
+                                                   int *__restrict__ counts, int *__restrict__ pairs,
+                                                   unsigned long long *__restrict__ npairs, long long cap)
+{
+    __shared__ double sU[6 * kMaxTypes * kMaxTypes];
+    __shared__ double lds[kBlock];
+    for (int t = threadIdx.x; t < 6 * nT * nT; t += kBlock) sU[t] = uconst[t];
+    __syncthreads();
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    double accu = 0.0;
+    if (s < n) {
+        const uint32_t fwd = forward_mask();
+        const int nx = box->nc[0], ny = box->nc[1], nz = box->nc[2];
+        const double Lx = box->L[0], Ly = box->L[1], Lz = box->L[2];
+        const double ax = box->scale[0], ay = box->scale[1], az = box->scale[2];
+        const Bead h = sb[s];
+        const uint32_t me = sidx[s];
+        const uint32_t k = skey[s];
+        const int kx = (int)(k % (uint32_t)nx);
+        const int ky = (int)((k / (uint32_t)nx) % (uint32_t)ny);
+        const int kz = (int)(k / (uint32_t)(nx * ny));
+        int cnt = 0;
+        for (int c = 0; c < 27; c++) {
+            int X0 = kx + c % 3 - 1, Y0 = ky + (c / 3) % 3 - 1, Z0 = kz + c / 9 - 1;
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            if (X0 < 0) { X0 += nx; sx = -Lx; } else if (X0 >= nx) { X0 -= nx; sx = Lx; }
+            if (Y0 < 0) { Y0 += ny; sy = -Ly; } else if (Y0 >= ny) { Y0 -= ny; sy = Ly; }
+            if (Z0 < 0) { Z0 += nz; sz = -Lz; } else if (Z0 >= nz) { Z0 -= nz; sz = Lz; }
+            const uint32_t kk = (uint32_t)(X0 + Y0 * nx + Z0 * nx * ny);
+            const int c0 = cstart[kk];
+            if (!((unsigned)c0 < (unsigned)n && skey[c0] == kk)) continue;
+            const int c1 = cend[kk];
+            const bool own = (c == 13) || ((fwd >> c) & 1u);
+            for (int q = c0; q < c1; q++) {
+                if (q == s) continue;
+                const Bead o = sb[q];
+                // owner's particle minus the other, image shift on the non-owner
+                double dx, dy, dz;
+                double ox, oy, oz, px, py, pz;  // owner-side, other-side positions (imaged)
+                if (own) {
+                    ox = h.x; oy = h.y; oz = h.z;
+                    px = o.x + sx; py = o.y + sy; pz = o.z + sz;
+                } else {
+                    ox = o.x; oy = o.y; oz = o.z;
+                    px = h.x - sx; py = h.y - sy; pz = h.z - sz;
+                }
+                dx = ox - px; dy = oy - py; dz = oz - pz;
+                const double r2 = sq3(dx, dy, dz);
+                const int t1 = own ? h.type : o.type, t2 = own ? o.type : h.type;
+                const double *C = sU + 6 * (t1 * nT + t2);
+                if (mode == GATHER_U) {
+                    if (r2 < cut2) accu += 0.5 * pair_u(r2, C);
+                } else if (mode == GATHER_DU) {
+                    const double oldP = (r2 < cut2) ? pair_u(r2, C) : 0.0;
+                    const double ex = ox * ax - px * ax, ey = oy * ay - py * ay, ez = oz * az - pz * az;
+                    const double r2n = sq3(ex, ey, ez);
+                    const double newP = (r2n < cut2) ? pair_u(r2n, C) : 0.0;
+                    accu += 0.5 * (oldP - newP);
+                } else if (r2 < cut2) {
+                    cnt++;
+                    if (mode == GATHER_LIST && me < sidx[q]) {
+                        const unsigned long long slot = atomicAdd(npairs, 1ull);
+                        if ((long long)slot < cap) {
+                            pairs[2 * slot] = (int)me;
+                            pairs[2 * slot + 1] = (int)sidx[q];
+                        }
+                    }
+                }
+            }
+        }
+        if (mode == GATHER_COUNT) counts[me] = cnt;
+    }
+    if (mode == GATHER_U || mode == GATHER_DU) {
+        const double b = block_sum(accu, lds);
+        if (threadIdx.x == 0) partial[blockIdx.x] = b;
+    }
+}
+
+// CHAIN energies in FP64 (harmonicP MD.h:422-431, bendP MD.h:769-789, walk
+// system.h:2487-2568 / dPotential system.h:3280-3384), one thread per chain.
+__device__ inline void mi3(const Bead &p, const Bead &q, const double *L, double d[3])
+{
+    d[0] = p.x - q.x; d[1] = p.y - q.y; d[2] = p.z - q.z;
+    for (int k = 0; k < 3; k++) {
+        if (d[k] > L[k] / 2.0) d[k] -= L[k];
+        if (d[k] < -L[k] / 2.0) d[k] += L[k];
+    }
+}
+
+__device__ inline double harm_p(const double d[3], double l0, double kb)
+{
+    const double dr = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double p = dr - l0;
+    return 0.5 * kb * p * p;
+}
+
+__device__ inline double bend_p(const double a[3], const double b[3], double c0, double kt)
+{
+    const double ra = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    const double rb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    const double cost = (a[0] / ra) * (b[0] / rb) + (a[1] / ra) * (b[1] / rb) + (a[2] / ra) * (b[2] / rb);
+    const double p = c0 - cost;
+    return kt * p * p * 0.5;
+}
+
+__global__ __launch_bounds__(kBlock) void k_chain_energy(int nch, const int2 *__restrict__ chains,
+                                                         const double4 *__restrict__ cconst,
+                                                         const Bead *__restrict__ beads,
+                                                         const DevBox *__restrict__ box, int delta,
+                                                         double *__restrict__ partial)
+{
+    __shared__ double lds[kBlock];
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    double accu = 0.0;
+    if (c < nch) {
+        const double L[3] = {box->L[0], box->L[1], box->L[2]};
+        const double S[3] = {box->scale[0], box->scale[1], box->scale[2]};
+        const int2 ch = chains[c];
+        const int start = ch.x, len = ch.y & 0xFFFF;
+        const double4 K = cconst[ch.y >> 16];
+        for (int l = start; l <= start + len - 3; l++) {
+            const bool last = (l == start + len - 3);
+            double da[3], db[3];
+            mi3(beads[l], beads[l + 1], L, da);
+            mi3(beads[l + 1], beads[l + 2], L, db);
+            double u = harm_p(da, K.x, K.y);
+            if (last) u += harm_p(db, K.x, K.y);
+            u += bend_p(da, db, K.z, K.w);
+            if (delta) {
+                for (int d = 0; d < 3; d++) { da[d] *= S[d]; db[d] *= S[d]; }
+                double un = harm_p(da, K.x, K.y);
+                if (last) un += harm_p(db, K.x, K.y);
+                un += bend_p(da, db, K.z, K.w);
+                accu += (u - un);
+            } else {
+                accu += u;
+            }
+        }
+    }
+    const double b = block_sum(accu, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = b;
+}
+
+// Kinetic::compute (dataCollection.h:667-673)
+__global__ __launch_bounds__(kBlock) void k_kinetic(int n, const float4 *__restrict__ vel,
+                                                    double *__restrict__ partial)
+{
+    __shared__ double lds[kBlock];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double e = 0.0;
+    if (i < n) {
+        const float4 v = vel[i];
+        const double x = v.x, y = v.y, z = v.z;
+        e = (x * x + y * y + z * z) / 2.0;
+    }
+    const double b = block_sum(e, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = b;
+}
+
+// ---------------------------------------------------------------------------------------
+// Monte-Carlo constant-tension move (MD.cpp:589-721), decided on the device.
+// ---------------------------------------------------------------------------------------
+__global__ void k_mc_propose(DevBox *box, unsigned long long seed, long long step, double dL)
+{
+    const uint4 o = philox(0u, (uint32_t)step, (uint32_t)((unsigned long long)step >> 32), TAG_MC,
+                           (uint32_t)seed, (uint32_t)(seed >> 32));
+    const double psi = psi64(o.x);
+    box->u_accept = ((o.y >> 5) * 67108864.0 + (o.z >> 6)) * (1.0 / 9007199254740992.0);
+    const double Lx = box->L[0], Ly = box->L[1], Lz = box->L[2];
+    const double fx = dL * psi, fy = fx;
+    const double fz = (Lx * Ly) / ((Lx + fx) * (Ly + fy));
+    box->fluct = fx;
+    box->newL[0] = Lx + fx;
+    box->newL[1] = Ly + fy;
+    box->newL[2] = Lz * fz;
+    box->scale[0] = box->newL[0] / Lx;
+    box->scale[1] = box->newL[1] / Ly;
+    box->scale[2] = box->newL[2] / Lz;
+    box->accept = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_mc_decide(DevBox *box, const double *__restrict__ pp, int np,
+                                                      const double *__restrict__ pc, int nc, double T,
+                                                      double tension, double dL, double cutoff)
+{
+    __shared__ double lds[kBlock];
+    const double dup = reduce_partials(pp, np, lds);
+    const double duc = reduce_partials(pc, nc, lds);
+    if (threadIdx.x != 0) return;
+    const double Lx = box->L[0], Ly = box->L[1];
+    double dP = dup;
+    dP += duc;
+    if (tension != 0) dP += tension * ((box->newL[0] * box->newL[1]) - (Lx * Ly));
+    double D = dP / T;
+    D = exp(D);
+    bool acc = (D >= box->u_accept) || (-dP <= 0);
+    int nc3[3];
+    long long cells = 1;
+    for (int d = 0; d < 3; d++) {
+        nc3[d] = (int)(box->newL[d] / cutoff);
+        cells *= nc3[d];
+    }
+    if (acc && (cells > box->capacity || nc3[0] < 3 || nc3[1] < 3 || nc3[2] < 3)) {
+        box->flags |= FLAG_CELL_OVERFLOW;
+        acc = false;
+    }
+    box->dU_pair = dup;
+    box->dU_chain = duc;
+    const double interval = dL / 101.0;
+    const long long bin = (long long)((box->fluct + dL) / interval);
+    if (acc) {
+        for (int d = 0; d < 3; d++) {
+            box->L[d] = box->newL[d];
+            box->nc[d] = nc3[d];
+            box->cs[d] = box->newL[d] / nc3[d];
+        }
+        box->accepted++;
+        if (bin >= 0 && bin < 256) box->hist_accept[bin] += 1.0;
+    } else if (bin >= 0 && bin < 256) {
+        box->hist_reject[bin] += 1.0;
+    }
+    box->accept = acc ? 1 : 0;
+    box->trials++;
+}
+
+__global__ __launch_bounds__(kBlock) void k_rescale(int n, Bead *__restrict__ beads, const DevBox *__restrict__ box)
+{
+    if (!box->accept) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    Bead b = beads[i];
+    b.x *= box->scale[0];
+    b.y *= box->scale[1];
+    b.z *= box->scale[2];
+    beads[i] = b;
+}
+
+inline int blocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------------------
+void launch_first(hipStream_t st, const Params &P, Bead *beads, float4 *vel, const float4 *acc, const DevBox *box,
+                  uint32_t *key, uint32_t *idx, int integrate)
+{
+    hipLaunchKernelGGL(k_first, dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, beads, vel, acc,
+                       const_cast<DevBox *>(box), key, idx, (float)(0.5 * P.dt), P.dt, integrate);
+}
+
+void launch_reorder(hipStream_t st, int n, const uint32_t *skey, const uint32_t *sidx, const Bead *beads,
+                    Bead *sbeads, int *cstart, int *cend)
+{
+    hipLaunchKernelGGL(k_reorder, dim3(blocks(n)), dim3(kBlock), 0, st, n, skey, sidx, beads, sbeads, cstart, cend);
+}
+
+void launch_pair_force(hipStream_t st, const Params &P, int chunk, const Bead *sbeads, const uint32_t *skey,
+                       const uint32_t *sidx, const int *cstart, const int *cend, const DevBox *box,
+                       const float *fconst, float4 *fpair)
+{
+    const long long waves = ((long long)P.n + chunk - 1) / chunk;
+    const int nb = (int)((waves + kPairWaves - 1) / kPairWaves);
+    hipLaunchKernelGGL(k_pair_force, dim3(nb), dim3(kPairWaves * kWave), 0, st, P.n, P.nT, chunk, nb,
+                       P.cutoff * P.cutoff, sbeads, skey, sidx, cstart, cend, box, fconst, fpair);
+}
+
+void launch_bonded_kick(hipStream_t st, const Params &P, const Bead *beads, const int2 *chain, const float4 *cconst,
+                        const float4 *fpair, float4 *vel, float4 *acc, const DevBox *box, long long step, int mode,
+                        float4 *fchain_out)
+{
+    const double sigma = sqrt((6.0 * P.temperature * P.gamma) / P.dt);
+    hipLaunchKernelGGL(k_bonded_kick, dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, beads, chain, cconst, fpair, vel,
+                       acc, box, P.seed, step, (float)P.gamma, (float)sigma, (float)(P.dt * 0.5), mode, fchain_out);
+}
+
+void launch_second(hipStream_t st, int n, const Bead *beads, const float4 *acc, float4 *vel, float halfdt)
+{
+    hipLaunchKernelGGL(k_second, dim3(blocks(n)), dim3(kBlock), 0, st, n, beads, acc, vel, halfdt);
+}
+
+int gather_blocks(int n) { return blocks(n > 0 ? n : 1); }
+
+void launch_pair_gather(hipStream_t st, const Params &P, int mode, const Bead *sbeads, const uint32_t *skey,
+                        const uint32_t *sidx, const int *cstart, const int *cend, const DevBox *box,
+                        const double *uconst, double *partial, int *counts, int *pairs, unsigned long long *npairs,
+                        long long cap)
+{
+    hipLaunchKernelGGL(k_gather, dim3(gather_blocks(P.n)), dim3(kBlock), 0, st, mode, P.n, P.nT,
+                       P.cutoff * P.cutoff, sbeads, skey, sidx, cstart, cend, box, uconst, partial, counts, pairs,
+                       npairs, cap);
+}
+
+int chain_blocks(int nchains) { return blocks(nchains > 0 ? nchains : 1); }
+
+void launch_chain_energy(hipStream_t st, int nchains, const int2 *chains, const double4 *cconst, const Bead *beads,
+                         const DevBox *box, int delta, double *partial)
+{
+    hipLaunchKernelGGL(k_chain_energy, dim3(chain_blocks(nchains)), dim3(kBlock), 0, st, nchains, chains, cconst,
+                       beads, box, delta, partial);
+}
+
+void launch_kinetic(hipStream_t st, int n, const float4 *vel, double *partial)
+{
+    hipLaunchKernelGGL(k_kinetic, dim3(gather_blocks(n)), dim3(kBlock), 0, st, n, vel, partial);
+}
+
+void launch_reduce_sum(hipStream_t st, const double *partial, int count, double *out)
+{
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, st, partial, count, out);
+}
+
+void launch_mc_propose(hipStream_t st, const Params &P, DevBox *box, long long step)
+{
+    hipLaunchKernelGGL(k_mc_propose, dim3(1), dim3(1), 0, st, box, P.seed, step, P.delta_lxy);
+}
+
+void launch_mc_decide(hipStream_t st, const Params &P, DevBox *box, const double *pp, int np, const double *pc,
+                      int nc)
+{
+    hipLaunchKernelGGL(k_mc_decide, dim3(1), dim3(kBlock), 0, st, box, pp, np, pc, nc, P.temperature, P.tension,
+                       P.delta_lxy, P.cutoff);
+}
+
+void launch_rescale(hipStream_t st, int n, Bead *beads, const DevBox *box)
+{
+    hipLaunchKernelGGL(k_rescale, dim3(blocks(n)), dim3(kBlock), 0, st, n, beads, box);
+}
+
+}  // namespace smdev

@@ -1,0 +1,221 @@
+"""Parity of the gfx950 path (through the C ABI) with the FP64 oracle on the same inputs.
+
+Bars: cell keys and the neighbour pair set bit-exact; FP64 energies and dU to 1e-10
+relative; step-0 pair forces (FP32 pair arithmetic on FP64 position differences) within
+1e-5 relative, measured two ways -- per particle |dF_i| / max(|F_i|, 0.05 rms|F|) and
+globally ||dF||_2 / ||F||_2.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import softmold_amd as sm
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs the MI355X")]
+
+FORCE_TOL = 1e-5      # per-particle, floored
+FORCE_TOL_L2 = 1e-6   # global L2
+
+
+@pytest.fixture(scope="module", autouse=True)
+def oracle_threads():
+    O.set_threads(8)
+    yield
+
+
+_cache = {}
+
+
+def system(name):
+    if name not in _cache:
+        gen = {
+            "L500": lambda: sm.Mpd.liposome(1234, 500, 3.45),
+            "B400": lambda: sm.Mpd.bilayer(1234, 400, 3.45),
+            "L5000": lambda: sm.Mpd.liposome(1234, 5000, 3.45),     # 15,000 beads
+            "B4000": lambda: sm.Mpd.bilayer(1234, 4000, 3.45),      # 12,000 beads
+            "L80000": lambda: sm.Mpd.liposome(1234, 80000, 3.45),   # 240,000 beads
+        }[name]
+        _cache[name] = gen().system()
+    return _cache[name]
+
+
+def oracle_of(d, **kw):
+    args = dict(cutoff=d["cutoff"], dt=d["dt"], gamma=d["gamma"], temp=d["temperature"],
+                delta_lxy=d["delta_lxy"], tension=d["tension"], seed=d["seed"])
+    args.update(kw)
+    return O.System(d["pos"], d["type"], d["vel"], d["box"], d["fconst"], d["uconst"], d["n_types"],
+                    d["chain"], d["chain_const"], **args)
+
+
+def force_errors(f, ref):
+    mag = np.linalg.norm(ref, axis=1)
+    floor = 0.05 * np.sqrt((mag ** 2).mean())
+    per = np.linalg.norm(f - ref, axis=1) / np.maximum(mag, floor)
+    l2 = np.linalg.norm(f - ref) / np.linalg.norm(ref)
+    return per, l2
+
+
+@pytest.mark.parametrize("name", ["L500", "B400", "L5000", "B4000", "L80000"])
+def test_cell_keys_bit_exact(name):
+    d = system(name)
+    print(f"case cell_keys {name} N={len(d['type'])}", flush=True)
+    s = sm.Solver.from_system(d)
+    np.testing.assert_array_equal(s.cell_keys(), oracle_of(d).cell_keys())
+
+
+@pytest.mark.parametrize("name", ["L500", "B400", "B4000"])
+def test_pair_set_bit_exact(name):
+    d = system(name)
+    print(f"case pair_set {name}", flush=True)
+    s = sm.Solver.from_system(d)
+    ours = s.pair_list()
+    ref = oracle_of(d).pair_list()
+    assert len(ours) == len(ref)
+    assert set(map(tuple, ours.tolist())) == set(map(tuple, ref.tolist()))
+
+
+@pytest.mark.parametrize("name", ["L5000", "L80000"])
+def test_neighbour_counts_bit_exact(name):
+    d = system(name)
+    print(f"case counts {name}", flush=True)
+    s = sm.Solver.from_system(d)
+    c, total = s.pair_count()
+    rc = oracle_of(d).pair_count()
+    np.testing.assert_array_equal(c, rc)
+    assert total == rc.sum() // 2
+
+
+@pytest.mark.parametrize("name", ["L500", "B400", "L5000", "B4000", "L80000"])
+def test_step0_pair_forces_fp32_vs_fp64(name):
+    d = system(name)
+    print(f"case forces {name}", flush=True)
+    s = sm.Solver.from_system(d)
+    fp, _ = s.compute_forces()
+    ref = oracle_of(d).pair_force()
+    per, l2 = force_errors(fp, ref)
+    print(f"{name}: per-particle max {per.max():.3e} p99.9 {np.quantile(per, 0.999):.3e} L2 {l2:.3e}")
+    assert l2 < FORCE_TOL_L2
+    assert per.max() < FORCE_TOL
+    # Newton's third law holds to FP32 rounding
+    assert np.linalg.norm(fp.sum(0)) < 1e-5 * np.sqrt(len(fp)) * np.abs(ref).max()
+
+
+def test_chain_forces_on_deformed_chains():
+    d = dict(system("B4000"))
+    rng = np.random.default_rng(5)
+    pos = d["pos"] + rng.normal(0, 0.05, d["pos"].shape)
+    pos = np.mod(pos, d["box"])
+    d["pos"] = pos
+    s = sm.Solver.from_system(d)
+    _, fc = s.compute_forces()
+    ref = oracle_of(d).chain_force()
+    per, l2 = force_errors(fc, ref)
+    print(f"chain: per max {per.max():.3e} L2 {l2:.3e}")
+    assert l2 < 1e-6 and per.max() < 1e-5
+
+
+@pytest.mark.parametrize("name", ["L500", "B400", "L5000", "B4000"])
+def test_energies_fp64(name):
+    d = system(name)
+    s = sm.Solver.from_system(d)
+    U_pair, U_chain, K = s.measure()
+    o = oracle_of(d)
+    assert U_pair == pytest.approx(o.pair_potential(), rel=1e-12)
+    assert U_chain == pytest.approx(o.chain_potential(), rel=1e-6, abs=1e-12)
+    # velocities live in FP32 on the device
+    assert K == pytest.approx(o.kinetic(), rel=1e-6)
+
+
+def test_cfg1_known_answer():
+    s = sm.Solver.from_system(system("L5000"))
+    U_pair, _, K = s.measure()
+    assert U_pair == pytest.approx(-388321.0139620534, rel=1e-12)
+    assert K == pytest.approx(1.5 * 15000 * 3.0, rel=1e-6)
+
+
+@pytest.mark.parametrize("name", ["B400", "B4000", "L5000"])
+def test_potential_delta_fp64(name):
+    d = system(name)
+    s = sm.Solver.from_system(d)
+    o = oracle_of(d)
+    for scale in ([1.0001, 1.0001, 1 / 1.0001 ** 2], [0.999, 0.999, 1 / 0.999 ** 2]):
+        dp, dc = s.potential_delta(scale)
+        assert dp == pytest.approx(o.pair_dpotential(scale), rel=1e-10, abs=1e-10)
+        assert dc == pytest.approx(o.chain_dpotential(scale), rel=1e-8, abs=1e-12)
+
+
+def test_dense_cells_and_boundary_particles():
+    """Cells with more than 64 particles and stencils beyond one LDS pass; particles at
+    exactly 0 and L."""
+    rng = np.random.default_rng(9)
+    n = 3000
+    box = np.array([12.0, 12.0, 12.0])
+    pos = rng.uniform(0, 1, (n, 3)) * np.array([12.0, 12.0, 3.0])   # packed into z < 3
+    pos[0] = [0.0, 0.0, 0.0]
+    pos[1] = [12.0, 12.0, 12.0]
+    types = rng.integers(2, 4, n).astype(np.int32)
+    d = dict(system("L500"))
+    d.update(pos=pos, type=types, vel=np.zeros((n, 3)), box=box,
+             chain=np.zeros((0, 3), np.int32), chain_const=np.zeros((0, 4)))
+    s = sm.Solver.from_system(d)
+    o = oracle_of(d)
+    np.testing.assert_array_equal(s.cell_keys(), o.cell_keys())
+    assert set(map(tuple, s.pair_list().tolist())) == set(map(tuple, o.pair_list().tolist()))
+    fp, _ = s.compute_forces()
+    per, l2 = force_errors(fp, o.pair_force())
+    print(f"dense: per max {per.max():.3e} L2 {l2:.3e}")
+    assert l2 < FORCE_TOL_L2 and per.max() < FORCE_TOL
+
+
+def test_deterministic_runs():
+    d = system("B4000")
+    out = []
+    for _ in range(2):
+        s = sm.Solver.from_system(d, delta_lxy=0.01, tension=0.5, seed=42)
+        s.init_forces()
+        s.run(0, 25)
+        out.append(s.download())
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_step_phases_equal_run():
+    d = system("B400")
+    a = sm.Solver.from_system(d, delta_lxy=0.01, tension=1.0, seed=7)
+    b = sm.Solver.from_system(d, delta_lxy=0.01, tension=1.0, seed=7)
+    a.init_forces()
+    b.init_forces()
+    a.run(0, 17)
+    for i in range(17):
+        b.step_first(i)
+        b.step_finish(i, measure=(i % 5 == 0))
+    for x, y in zip(a.download(), b.download()):
+        np.testing.assert_array_equal(x, y)
+    assert a.mc_stats() == b.mc_stats()
+
+
+@pytest.mark.parametrize("name,steps", [("B400", 24), ("L500", 12)])
+def test_trajectory_tracks_oracle_with_same_noise(name, steps):
+    """Same initial state and the same counter-RNG noise: the FP32 GPU trajectory stays
+    within FP32 drift of the FP64 oracle, and every MC decision is the same."""
+    d = system(name)
+    kw = dict(delta_lxy=0.01, tension=1.0, seed=99) if name == "B400" else dict(seed=99)
+    s = sm.Solver.from_system(d, **kw)
+    o = oracle_of(d, **kw)
+    md = O.MD(o, rng_mode=1)
+    s.init_forces()
+    md.init_forces()
+    s.run(0, steps)
+    acc = md.run(0, steps - 1)
+    pos, vel, _, box = s.download()
+    if name == "B400":
+        t, a = s.mc_stats()
+        assert t == md.trials() == (steps - 1) // 8 and a == acc
+        np.testing.assert_allclose(box, o.box, rtol=1e-14)
+    dp = pos - o.pos
+    dp -= box * np.round(dp / box)
+    print(f"{name}: max |dp| {np.abs(dp).max():.3e}  max |dv| {np.abs(vel - o.vel).max():.3e}")
+    assert np.abs(dp).max() < 1e-4
+    assert np.abs(vel - o.vel).max() < 1e-2

@@ -1,0 +1,98 @@
+"""CPU tests of the product's host side: the C-ABI library loads and exports every symbol
+the header declares, the generators reproduce the reference's `.mpd` bytes, and the
+`.mpd` reader/writer keep the reference's format and error behaviour.  No GPU calls."""
+import hashlib
+import json
+import os
+import re
+import tempfile
+
+import numpy as np
+import pytest
+
+import softmold_amd as sm
+from conftest import GOLDEN, ROOT
+
+HEADER = os.path.join(ROOT, "include", "softmold_hip.h")
+
+
+def test_library_exports_every_declared_symbol():
+    text = open(HEADER).read()
+    declared = set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(sm_\w+)\s*\(", text, re.M))
+    assert declared == set(sm.EXPORTS)
+    L = sm.lib()
+    for name in declared:
+        assert hasattr(L, name), name
+    assert b"gfx950" in L.sm_version()
+
+
+def _sha(path):
+    return hashlib.sha256(open(path, "rb").read()).hexdigest()
+
+
+@pytest.mark.parametrize("name", ["L500", "B400", "L5000", "B4000", "L2000s7", "L80000"])
+def test_generators_match_reference_bytes(name, tmp_path):
+    rec = json.load(open(os.path.join(GOLDEN, "gen_sha256.json")))[name]
+    kind, seed, n, rho = rec["argv"][0], int(rec["argv"][1]), int(rec["argv"][2]), float(rec["argv"][3])
+    m = sm.Mpd.liposome(seed, n, rho) if kind == "liposome" else sm.Mpd.bilayer(seed, n, rho)
+    out = str(tmp_path / name)
+    m.write(out)
+    assert _sha(out + ".mpd") == rec["sha256"]
+
+
+def test_mpd_round_trip_is_byte_identical(golden_traj, tmp_path):
+    for name in ("L500", "B400t"):
+        src = tmp_path / f"{name}.mpd"
+        src.write_bytes(golden_traj[name + "/final_mpd"].tobytes())
+        m = sm.Mpd.read(str(tmp_path / name))
+        m.write(str(tmp_path / (name + "_rt")))
+        assert (tmp_path / f"{name}_rt.mpd").read_bytes() == src.read_bytes()
+
+
+def test_mpd_system_view(tmp_path):
+    m = sm.Mpd.bilayer(1234, 400, 3.45)
+    d = m.system()
+    assert d["pos"].shape == (1176, 3) and d["n_types"] == 6
+    assert d["chain"].tolist() == [[0, 392, 3]]
+    assert d["delta_lxy"] == 0.01 and d["tension"] == 0.0
+    assert m["deltaLXY"] == 0.01 and m["nParticles"] == 1176
+    m["tension"] = 0.5
+    m.write(str(tmp_path / "t"))
+    assert "\ntension 0.5 \n" in (tmp_path / "t.mpd").read_text()
+
+
+def test_mpd_errors_mirror_reference(tmp_path):
+    with pytest.raises(sm.SmError) as e:
+        sm.Mpd.read(str(tmp_path / "missing"))
+    assert e.value.code == sm.SM_ERR_IO
+    good = sm.Mpd.liposome(1, 50, 3.45)
+    good.write(str(tmp_path / "g"))
+    text = (tmp_path / "g.mpd").read_text()
+    (tmp_path / "bad.mpd").write_text(text.replace("cutoff 2", "cutof 2"))
+    with pytest.raises(sm.SmError, match="is not a recognized command"):
+        sm.Mpd.read(str(tmp_path / "bad"))
+    (tmp_path / "bad2.mpd").write_text(text.replace("seed 1", "seed x"))
+    with pytest.raises(sm.SmError, match="not a numerical type"):
+        sm.Mpd.read(str(tmp_path / "bad2"))
+    # a particle outside the box (Blob::errorChecking, system.h:452-469)
+    lines = text.split("\n")
+    i = next(k for k, l in enumerate(lines) if l.startswith("positions")) + 1
+    f = lines[i].split()
+    lines[i] = " " + " ".join([f[0], "401.5", f[2], f[3]])
+    (tmp_path / "bad3.mpd").write_text("\n".join(lines))
+    with pytest.raises(sm.SmError, match="out of bounds"):
+        sm.Mpd.read(str(tmp_path / "bad3"))
+
+
+def test_unsupported_molecules_are_refused(tmp_path):
+    m = sm.Mpd.liposome(1, 50, 3.45)
+    m.write(str(tmp_path / "g"))
+    text = (tmp_path / "g.mpd").read_text()
+    # turn the CHAIN record into a BOND record over the first two beads
+    text = re.sub(r"\nmolecule \n8\t 1\n 0.7  100  1  100 \n 0  50  3 \n",
+                  "\nmolecule \n6\t 1\n 0.7  100 \n 0  1 \n", text)
+    (tmp_path / "b.mpd").write_text(text)
+    b = sm.Mpd.read(str(tmp_path / "b"))
+    with pytest.raises(sm.SmError) as e:
+        b.system()
+    assert e.value.code == sm.SM_ERR_UNSUPPORTED
