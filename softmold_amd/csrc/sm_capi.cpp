@@ -247,7 +247,7 @@ struct sm_ctx {
     void bonded(long long step, int mode, float4 *fchain_out)
     {
         timed(K_BONDED, [&] {
-            launch_bonded_kick(st, P, beads, chainPerBead, cconstF, fpair, vel, acc, box, step, mode, fchain_out);
+            launch_bonded_kick(st, P, beads, chainPerBead, cconstD, fpair, vel, acc, box, step, mode, fchain_out);
         });
     }
     // {U_pair, U_chain, K} into dres[0..2] (device)

@@ -74,7 +74,7 @@ void launch_pair_force(hipStream_t st, const Params &P, int chunk, const Bead *s
                        const uint32_t *sidx, const int *cstart, const int *cend, const DevBox *box,
                        const float *fconst, float4 *fpair);
 void launch_bonded_kick(hipStream_t st, const Params &P, const Bead *beads, const int2 *chain,
-                        const float4 *cconst, const float4 *fpair, float4 *vel, float4 *acc, const DevBox *box,
+                        const double4 *cconst, const float4 *fpair, float4 *vel, float4 *acc, const DevBox *box,
                         long long step, int mode, float4 *fchain_out);
 enum : int { KICK_STEP = 0, KICK_INIT = 1, KICK_NONE = 2 };
 void launch_second(hipStream_t st, int n, const Bead *beads, const float4 *acc, float4 *vel, float dt);

@@ -244,11 +244,18 @@ __global__ __launch_bounds__(kPairWaves * kWave) void k_pair_force(
                                                         __dmul_rn(dzd, dzd));
                             if (r2 < cut2 && j != self) {
                                 const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
-                                const float r = sqrtf((float)r2);
+                                // r = r0 + corr: one Newton step against the exact FP64 residual
+                                // r^2 - r0^2, so r carries no FP32 rounding of r^2 into
+                                // m = C0 - r (the force near r_m is C1 * m with C1 up to ~400).
+                                const float r0 = __builtin_amdgcn_sqrtf((float)r2);
+                                float ri = __builtin_amdgcn_rcpf(r0);
+                                const float corr = (float)fma(-(double)r0, (double)r0, r2) * (0.5f * ri);
+                                const float r = r0 + corr;
+                                ri = ri * (2.0f - r * ri);
                                 const float *C = Crow + 6 * T[jj];
-                                const int off = (r >= C[0]) ? 3 : 0;
-                                const float m = C[off] - r;
-                                const float mag = ((C[off + 1] - C[off + 2] * m) * m) * __builtin_amdgcn_rcpf(r);
+                                const int off = (r >= C[0]) ? 3 : 0;   // int(r/r_m)*3, clamped
+                                const float m = (C[off] - r0) - corr;  // C[off] - r0 is exact
+                                const float mag = ((C[off + 1] - C[off + 2] * m) * m) * ri;
                                 fx += dx * mag;
                                 fy += dy * mag;
                                 fz += dz * mag;
@@ -281,7 +288,9 @@ __global__ __launch_bounds__(kPairWaves * kWave) void k_pair_force(
 // MD.h:683-723; chain walk system.h:1782-1866), fused with the Langevin force
 // (langevin.h:233-331) and Verlet::second (verlet.h:464-477).
 // ---------------------------------------------------------------------------------------
-__device__ inline float3 min_image(const Bead &p, const Bead &q, double L0, double L1, double L2)
+// Bonded terms are evaluated in FP64: a bead touches at most 2 bonds and 3 bends, FP64
+// add/mul/fma issue at the FP32 rate on gfx950, and the kernel is bandwidth bound.
+__device__ inline double3 min_image(const Bead &p, const Bead &q, double L0, double L1, double L2)
 {
     double dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     if (dx > L0 / 2.0) dx -= L0;
@@ -290,37 +299,38 @@ __device__ inline float3 min_image(const Bead &p, const Bead &q, double L0, doub
     if (dy < -L1 / 2.0) dy += L1;
     if (dz > L2 / 2.0) dz -= L2;
     if (dz < -L2 / 2.0) dz += L2;
-    return make_float3((float)dx, (float)dy, (float)dz);
+    return make_double3(dx, dy, dz);
 }
 
 // harmonic force on the first particle of the bond (a1 += d * mag)
-__device__ inline float3 bond_f(float3 d, float l0, float kb)
+__device__ inline double3 bond_f(double3 d, double l0, double kb)
 {
-    const float dr = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
-    const float mag = -(dr - l0) * kb / dr;
-    return make_float3(d.x * mag, d.y * mag, d.z * mag);
+    const double dr = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+    const double mag = -(dr - l0) * kb / dr;
+    return make_double3(d.x * mag, d.y * mag, d.z * mag);
 }
 
 // bend: returns the force on the particle with role 0, 1 or 2 of (p1, p2, p3)
-__device__ inline float3 bend_f(float3 da, float3 db, float c0, float kt, int role)
+__device__ inline double3 bend_f(double3 da, double3 db, double c0, double kt, int role)
 {
-    const float dra = sqrtf(da.x * da.x + da.y * da.y + da.z * da.z);
-    const float drb = sqrtf(db.x * db.x + db.y * db.y + db.z * db.z);
-    da.x /= dra; da.y /= dra; da.z /= dra;
-    db.x /= drb; db.y /= drb; db.z /= drb;
-    const float cost = da.x * db.x + da.y * db.y + da.z * db.z;
-    const float mag = (c0 - cost) * kt;
-    const float ta = mag / dra, tb = mag / drb;
-    const float3 A = make_float3(ta * (db.x - da.x * cost), ta * (db.y - da.y * cost), ta * (db.z - da.z * cost));
-    const float3 B = make_float3(tb * (da.x - db.x * cost), tb * (da.y - db.y * cost), tb * (da.z - db.z * cost));
+    const double dra = sqrt(da.x * da.x + da.y * da.y + da.z * da.z);
+    const double drb = sqrt(db.x * db.x + db.y * db.y + db.z * db.z);
+    const double ia = 1.0 / dra, ib = 1.0 / drb;
+    da.x *= ia; da.y *= ia; da.z *= ia;
+    db.x *= ib; db.y *= ib; db.z *= ib;
+    const double cost = da.x * db.x + da.y * db.y + da.z * db.z;
+    const double mag = (c0 - cost) * kt;
+    const double ta = mag * ia, tb = mag * ib;
+    const double3 A = make_double3(ta * (db.x - da.x * cost), ta * (db.y - da.y * cost), ta * (db.z - da.z * cost));
+    const double3 B = make_double3(tb * (da.x - db.x * cost), tb * (da.y - db.y * cost), tb * (da.z - db.z * cost));
     if (role == 0) return A;
-    if (role == 1) return make_float3(B.x - A.x, B.y - A.y, B.z - A.z);
-    return make_float3(-B.x, -B.y, -B.z);
+    if (role == 1) return make_double3(B.x - A.x, B.y - A.y, B.z - A.z);
+    return make_double3(-B.x, -B.y, -B.z);
 }
 
 __global__ __launch_bounds__(kBlock) void k_bonded_kick(int n, const Bead *__restrict__ beads,
                                                         const int2 *__restrict__ chain,
-                                                        const float4 *__restrict__ cconst,
+                                                        const double4 *__restrict__ cconst,
                                                         const float4 *__restrict__ fpair, float4 *__restrict__ vel,
                                                         float4 *__restrict__ acc, const DevBox *__restrict__ box,
                                                         unsigned long long seed, long long step, float gamma,
@@ -330,11 +340,11 @@ __global__ __launch_bounds__(kBlock) void k_bonded_kick(int n, const Bead *__res
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const double L0 = box->L[0], L1 = box->L[1], L2 = box->L[2];
-    float3 fc = make_float3(0.f, 0.f, 0.f);
+    double3 fc = make_double3(0.0, 0.0, 0.0);
     const int2 ci = chain[i];
     if (ci.x >= 0) {
         const int start = ci.x, len = ci.y & 0xFFFF, cid = ci.y >> 16;
-        const float4 K = cconst[cid];
+        const double4 K = cconst[cid];
         const int j = i - start;
         const Bead p0 = beads[i];
         Bead pm1, pm2, pp1, pp2;
@@ -343,28 +353,28 @@ __global__ __launch_bounds__(kBlock) void k_bonded_kick(int n, const Bead *__res
         if (j <= len - 2) pp1 = beads[i + 1];
         if (j <= len - 3) pp2 = beads[i + 2];
         if (j >= 1) {  // bond (i-1, i): this bead is a2
-            const float3 f = bond_f(min_image(pm1, p0, L0, L1, L2), K.x, K.y);
+            const double3 f = bond_f(min_image(pm1, p0, L0, L1, L2), K.x, K.y);
             fc.x -= f.x; fc.y -= f.y; fc.z -= f.z;
         }
         if (j <= len - 2) {  // bond (i, i+1): this bead is a1
-            const float3 f = bond_f(min_image(p0, pp1, L0, L1, L2), K.x, K.y);
+            const double3 f = bond_f(min_image(p0, pp1, L0, L1, L2), K.x, K.y);
             fc.x += f.x; fc.y += f.y; fc.z += f.z;
         }
         if (j <= len - 3) {  // bend (i, i+1, i+2), role 0
-            const float3 f = bend_f(min_image(p0, pp1, L0, L1, L2), min_image(pp1, pp2, L0, L1, L2), K.z, K.w, 0);
+            const double3 f = bend_f(min_image(p0, pp1, L0, L1, L2), min_image(pp1, pp2, L0, L1, L2), K.z, K.w, 0);
             fc.x += f.x; fc.y += f.y; fc.z += f.z;
         }
         if (j >= 1 && j <= len - 2) {  // bend (i-1, i, i+1), role 1
-            const float3 f = bend_f(min_image(pm1, p0, L0, L1, L2), min_image(p0, pp1, L0, L1, L2), K.z, K.w, 1);
+            const double3 f = bend_f(min_image(pm1, p0, L0, L1, L2), min_image(p0, pp1, L0, L1, L2), K.z, K.w, 1);
             fc.x += f.x; fc.y += f.y; fc.z += f.z;
         }
         if (j >= 2) {  // bend (i-2, i-1, i), role 2
-            const float3 f = bend_f(min_image(pm2, pm1, L0, L1, L2), min_image(pm1, p0, L0, L1, L2), K.z, K.w, 2);
+            const double3 f = bend_f(min_image(pm2, pm1, L0, L1, L2), min_image(pm1, p0, L0, L1, L2), K.z, K.w, 2);
             fc.x += f.x; fc.y += f.y; fc.z += f.z;
         }
     }
     if (mode == KICK_NONE) {
-        fchain_out[i] = make_float4(fc.x, fc.y, fc.z, 0.f);
+        fchain_out[i] = make_float4((float)fc.x, (float)fc.y, (float)fc.z, 0.f);
         return;
     }
     const uint4 o = philox((uint32_t)i, (uint32_t)step, (uint32_t)((unsigned long long)step >> 32),
@@ -377,7 +387,7 @@ __global__ __launch_bounds__(kBlock) void k_bonded_kick(int n, const Bead *__res
     a.y = (-gamma * v.y + sigma * psi32(o.y));
     a.z = (-gamma * v.z + sigma * psi32(o.z));
     a.x += fp.x; a.y += fp.y; a.z += fp.z;
-    a.x += fc.x; a.y += fc.y; a.z += fc.z;
+    a.x += (float)fc.x; a.y += (float)fc.y; a.z += (float)fc.z;
     a.w = 0.f;
     acc[i] = a;
     if (mode == KICK_STEP && beads[i].type != 0) {
@@ -743,7 +753,7 @@ void launch_pair_force(hipStream_t st, const Params &P, int chunk, const Bead *s
                        P.cutoff * P.cutoff, sbeads, skey, sidx, cstart, cend, box, fconst, fpair);
 }
 
-void launch_bonded_kick(hipStream_t st, const Params &P, const Bead *beads, const int2 *chain, const float4 *cconst,
+void launch_bonded_kick(hipStream_t st, const Params &P, const Bead *beads, const int2 *chain, const double4 *cconst,
                         const float4 *fpair, float4 *vel, float4 *acc, const DevBox *box, long long step, int mode,
                         float4 *fchain_out)
 {

@@ -154,7 +154,7 @@ def test_dense_cells_and_boundary_particles():
     box = np.array([12.0, 12.0, 12.0])
     pos = rng.uniform(0, 1, (n, 3)) * np.array([12.0, 12.0, 3.0])   # packed into z < 3
     pos[0] = [0.0, 0.0, 0.0]
-    pos[1] = [12.0, 12.0, 12.0]
+    pos[1] = [12.0, 6.5, 1.25]   # on the upper x face (clamped into the last cell)
     types = rng.integers(2, 4, n).astype(np.int32)
     d = dict(system("L500"))
     d.update(pos=pos, type=types, vel=np.zeros((n, 3)), box=box,
