@@ -1,6 +1,7 @@
 """Throughput of the MI355X MD-step solver on one GPU.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py --calib-copy      # plain 512 MiB device copies, for HBM counter scaling
 
 Workload: the 80,000-lipid vesicle (240,000 beads) from the liposome generator (seed 1234,
 areal density 3.45, box 400^3, dt 0.02, T 3, gamma 1, tension 0 so no box moves), built
@@ -34,6 +35,9 @@ METRIC = "particle-steps/s at 1 GPU, plain vs batched FP32, and % of HBM rooflin
 HBM_PEAK_GBS = 8000.0
 VALU_FP32_PEAK_TFLOPS = 157.3
 SEED, NLIPIDS, DENSITY = 1234, 80000, 3.45
+# HBM bytes per pair_force launch on this workload, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE runs scaled by the same counters on a copy of known size (bench.py --calib-copy)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pair_force_vesicle240k_traffic.json")
 
 
 def log(*a):
@@ -108,16 +112,43 @@ def cpu_baseline(n_beads: int) -> dict:
         shutil.rmtree(work, ignore_errors=True)
 
 
+def calib_copy():
+    """Five copies of a 512 MiB buffer: the reference dispatch for scaling HBM counters."""
+    import torch
+
+    torch.cuda.set_device(0)
+    nbytes = 512 << 20
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "copies": 5}), flush=True)
+
+
+def traffic() -> float | None:
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        return float(t["bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true")
     args = ap.parse_args()
+    if args.calib_copy:
+        calib_copy()
+        return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(replicas would shard one subset per GPU; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: the solver has no multi-GPU path "
+              "(a replica sweep would shard one replica subset per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -174,13 +205,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 pair arithmetic, f64 positions/energies",
+        "dtype": "f32",
+        "precision": "f32 pair arithmetic; f64 positions, bonded forces, energies and dU",
         "data": "synthetic (liposome generator output, seed 1234)",
         "config": {"workload": "plain solver, vesicle 80000 lipids = 240000 beads, box 400^3, "
                                "dt 0.02, T 3, gamma 1, tension 0",
                    "n_beads": n, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic(),
                      "kernel": "pair_force", "kernel_avg_us": pair_ms * 1e3,
                      "algorithmic_bytes_per_launch": algo_bytes,
                      "valu_tflops": tflops, "valu_frac_fp32": tflops / VALU_FP32_PEAK_TFLOPS},

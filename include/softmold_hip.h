@@ -80,6 +80,10 @@ int sm_step_finish(sm_ctx *ctx, int64_t step, double *measure);
 int sm_run(sm_ctx *ctx, int64_t first_step, int32_t nsteps);
 /* MC statistics so far (MD.cpp:713-720,748). */
 int sm_mc_stats(sm_ctx *ctx, int64_t *trials, int64_t *accepted);
+/* Accepted/rejected trial histograms over the fluctuation (MD.cpp:106-116,708-718): bin
+ * width delta_lxy/101, bin b covers b*width - delta_lxy; n_out = int(2*101)+1 bins
+ * (0 when delta_lxy == 0); at most `cap` bins are copied. */
+int sm_mc_hist(sm_ctx *ctx, int32_t cap, double *accepted, double *rejected, int32_t *n_out);
 
 /* ---- measurement and parity hooks --------------------------------------------------- */
 /* {U_pair, U_chain, K} in FP64 at the current state (dataExtraction.h:840-842,942-948). */
@@ -98,6 +102,12 @@ int sm_pair_list(sm_ctx *ctx, int32_t *pairs, int64_t cap, int64_t *n_out);
 /* dU = U(old) - U(scaled) over the current pair set, FP64 (CellOpt::computeDPotential
  * cellOpt.h:1043-1180 and Blob::doChainDPotential system.h:3280-3384). */
 int sm_potential_delta(sm_ctx *ctx, const double scale[3], double *du_pair, double *du_chain);
+
+/* ---- launch shapes ------------------------------------------------------------------ */
+/* Set a launch-shape knob of this build; no counterpart in the reference (its launches are
+ * fixed, include/cuda/other.h:6-11).  Knobs: "pair_chunk" = sorted slots per wave64 in
+ * the pair-force kernel (1..4096). */
+int sm_tune(sm_ctx *ctx, const char *knob, int32_t value);
 
 /* ---- profiling support -------------------------------------------------------------- */
 /* Average device time (ms) of each kernel class over the launches since the last reset,

@@ -49,9 +49,9 @@ class SmSystem(C.Structure):
 EXPORTS = [
     "sm_last_error", "sm_version", "sm_create", "sm_destroy", "sm_sync", "sm_download_state",
     "sm_upload_state", "sm_set_temperature", "sm_init_forces", "sm_second_half",
-    "sm_step_first", "sm_step_finish", "sm_run", "sm_mc_stats", "sm_measure",
+    "sm_step_first", "sm_step_finish", "sm_run", "sm_mc_stats", "sm_mc_hist", "sm_measure",
     "sm_compute_forces", "sm_cell_keys", "sm_pair_count", "sm_pair_list",
-    "sm_potential_delta", "sm_profile_enable", "sm_profile_read", "sm_mpd_read",
+    "sm_potential_delta", "sm_tune", "sm_profile_enable", "sm_profile_read", "sm_mpd_read",
     "sm_mpd_write", "sm_mpd_free", "sm_mpd_system", "sm_mpd_get", "sm_mpd_set",
     "sm_mpd_set_state", "sm_gen_liposome", "sm_gen_bilayer",
 ]
@@ -90,12 +90,14 @@ def _declare(L):
         "sm_step_finish": (C.c_int, [vp, i64, P(d)]),
         "sm_run": (C.c_int, [vp, i64, i32]),
         "sm_mc_stats": (C.c_int, [vp, P(i64), P(i64)]),
+        "sm_mc_hist": (C.c_int, [vp, C.c_int32, P(d), P(d), P(i32)]),
         "sm_measure": (C.c_int, [vp, P(d)]),
         "sm_compute_forces": (C.c_int, [vp, P(d), P(d)]),
         "sm_cell_keys": (C.c_int, [vp, P(i32)]),
         "sm_pair_count": (C.c_int, [vp, P(i32), P(i64)]),
         "sm_pair_list": (C.c_int, [vp, P(i32), i64, P(i64)]),
         "sm_potential_delta": (C.c_int, [vp, P(d), P(d), P(d)]),
+        "sm_tune": (C.c_int, [vp, C.c_char_p, i32]),
         "sm_profile_enable": (C.c_int, [vp, i32]),
         "sm_profile_read": (C.c_int, [vp, i32, P(C.c_char_p), P(d), P(i64), P(i32)]),
         "sm_mpd_read": (C.c_int, [C.c_char_p, P(vp)]),
@@ -290,6 +292,12 @@ class Solver:
         _check(lib().sm_mc_stats(self.h, C.byref(t), C.byref(a)))
         return t.value, a.value
 
+    def mc_hist(self):
+        """(accepted, rejected) trial histograms over the box fluctuation."""
+        a, r, nb = np.zeros(256), np.zeros(256), C.c_int32()
+        _check(lib().sm_mc_hist(self.h, 256, _dp(a), _dp(r), C.byref(nb)))
+        return a[:nb.value].copy(), r[:nb.value].copy()
+
     # ---- state ----
     def download(self):
         n = self.n
@@ -339,6 +347,9 @@ class Solver:
         a, b = C.c_double(), C.c_double()
         _check(lib().sm_potential_delta(self.h, _dp(s), C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def tune(self, knob: str, value: int):
+        _check(lib().sm_tune(self.h, knob.encode(), int(value)))
 
     # ---- profiling ----
     def profile(self, on: bool):

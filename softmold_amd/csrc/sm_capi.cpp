@@ -105,6 +105,7 @@ struct sm_ctx {
     long long pairs_cap = 0;
     unsigned long long *dnp = nullptr;
     int chunk = 64;
+    int pair_variant = 0;
     int sorted_sel = 0;  // which sort buffer pair holds the sorted result
     // profiling
     bool prof = false;
@@ -187,6 +188,8 @@ struct sm_ctx {
         DevBox hb = read_box();
         if (hb.flags & FLAG_BAD_POSITION)
             throw ApiError(SM_ERR_STATE, "a particle position left the box or became non-finite");
+        if (hb.flags & FLAG_SORT_TIMEOUT)
+            throw ApiError(SM_ERR_STATE, "cell-key sort look-back timed out");
         if (hb.flags & FLAG_CELL_OVERFLOW)
             throw ApiError(SM_ERR_STATE, "an accepted box move would exceed the cell table or make fewer than 3 cells per axis");
     }
@@ -228,7 +231,7 @@ struct sm_ctx {
     void sort_and_reorder()
     {
         timed(K_SORT, [&] {
-            SortBuffers sb{k0, v0, k1, v1, hist, n, 0};
+            SortBuffers sb{k0, v0, k1, v1, hist, &box->flags, n, 0};
             sorted_sel = radix_sort(st, sb, sort_bits);
         });
         timed(K_REORDER, [&] { launch_reorder(st, n, skey(), sidx(), beads, sbeads, cstart, cend); });
@@ -241,7 +244,7 @@ struct sm_ctx {
     void pair_force()
     {
         timed(K_PAIR, [&] {
-            launch_pair_force(st, P, chunk, sbeads, skey(), sidx(), cstart, cend, box, fconstF, fpair);
+            launch_pair_force(st, P, chunk, pair_variant, sbeads, skey(), sidx(), cstart, cend, box, fconstF, fpair);
         });
     }
     void bonded(long long step, int mode, float4 *fchain_out)
@@ -254,19 +257,19 @@ struct sm_ctx {
     void measure_device()
     {
         timed(K_GATHER, [&] {
-            launch_pair_gather(st, P, GATHER_U, sbeads, skey(), sidx(), cstart, cend, box, uconstD, partA, nullptr,
+            launch_pair_gather(st, P, chunk, GATHER_U, sbeads, skey(), sidx(), cstart, cend, box, uconstD, partA, nullptr,
                                nullptr, nullptr, 0);
         });
         timed(K_CHAIN_E, [&] { launch_chain_energy(st, nchains, chainList, cconstD, beads, box, 0, partB); });
         timed(K_KIN, [&] { launch_kinetic(st, n, vel, partC); });
-        launch_reduce_sum(st, partA, gather_blocks(n), dres + 0);
+        launch_reduce_sum(st, partA, cells_blocks(n, chunk), dres + 0);
         launch_reduce_sum(st, partB, chain_blocks(nchains), dres + 1);
         launch_reduce_sum(st, partC, gather_blocks(n), dres + 2);
     }
     void delta_device()
     {
         timed(K_GATHER, [&] {
-            launch_pair_gather(st, P, GATHER_DU, sbeads, skey(), sidx(), cstart, cend, box, uconstD, partA, nullptr,
+            launch_pair_gather(st, P, chunk, GATHER_DU, sbeads, skey(), sidx(), cstart, cend, box, uconstD, partA, nullptr,
                                nullptr, nullptr, 0);
         });
         timed(K_CHAIN_E, [&] { launch_chain_energy(st, nchains, chainList, cconstD, beads, box, 1, partB); });
@@ -276,7 +279,7 @@ struct sm_ctx {
         timed(K_MC, [&] { launch_mc_propose(st, P, box, step); });
         delta_device();
         timed(K_MC, [&] {
-            launch_mc_decide(st, P, box, partA, gather_blocks(n), partB, chain_blocks(nchains));
+            launch_mc_decide(st, P, box, partA, cells_blocks(n, chunk), partB, chain_blocks(nchains));
             launch_rescale(st, n, beads, box);
         });
     }
@@ -427,7 +430,7 @@ int sm_create(const sm_system *s, sm_ctx **out)
         c->v0 = dalloc<uint32_t>(n);
         c->k1 = dalloc<uint32_t>(n);
         c->v1 = dalloc<uint32_t>(n);
-        c->hist = dalloc<uint32_t>(sort_hist_entries(n));
+        c->hist = dalloc<uint32_t>(sort_scratch_words(n, 4));
         c->box = dalloc<DevBox>(1);
         c->fconstF = dalloc<float>(fF.size());
         c->uconstD = dalloc<double>(fF.size());
@@ -436,7 +439,7 @@ int sm_create(const sm_system *s, sm_ctx **out)
         c->cconstF = dalloc<float4>(ccF.size());
         c->cconstD = dalloc<double4>(ccD.size());
         const int gb = gather_blocks(n);
-        c->partA = dalloc<double>(gb);
+        c->partA = dalloc<double>(cells_blocks(n, 1));
         c->partB = dalloc<double>(chain_blocks(c->nchains));
         c->partC = dalloc<double>(gb);
         c->dres = dalloc<double>(4);
@@ -466,7 +469,7 @@ int sm_create(const sm_system *s, sm_ctx **out)
             HIPCHK(hipMemcpyAsync(c->cconstD, ccD.data(), sizeof(double4) * ccD.size(), hipMemcpyHostToDevice, st));
         }
         // small systems: shorter chunks so the pair kernel still fills the chip
-        c->chunk = (n >= 200000) ? 64 : (n >= 50000 ? 32 : 16);
+        c->chunk = (n >= 100000) ? 16 : 8;
         c->ensure_capacity(0);
         HIPCHK(hipStreamSynchronize(st));
     });
@@ -497,11 +500,11 @@ int sm_download_state(sm_ctx *c, double *pos, double *vel, double *acc, double b
     return guarded([&] {
         if (!c) invalid("null context");
         const int n = c->n;
-        std::vector<Bead> hb(n);
-        std::vector<float4> hv(n), ha(n);
-        HIPCHK(hipMemcpyAsync(hb.data(), c->beads, sizeof(Bead) * n, hipMemcpyDeviceToHost, c->st));
-        HIPCHK(hipMemcpyAsync(hv.data(), c->vel, sizeof(float4) * n, hipMemcpyDeviceToHost, c->st));
-        HIPCHK(hipMemcpyAsync(ha.data(), c->acc, sizeof(float4) * n, hipMemcpyDeviceToHost, c->st));
+        std::vector<Bead> hb(pos ? n : 0);
+        std::vector<float4> hv(vel ? n : 0), ha(acc ? n : 0);
+        if (pos) HIPCHK(hipMemcpyAsync(hb.data(), c->beads, sizeof(Bead) * n, hipMemcpyDeviceToHost, c->st));
+        if (vel) HIPCHK(hipMemcpyAsync(hv.data(), c->vel, sizeof(float4) * n, hipMemcpyDeviceToHost, c->st));
+        if (acc) HIPCHK(hipMemcpyAsync(ha.data(), c->acc, sizeof(float4) * n, hipMemcpyDeviceToHost, c->st));
         DevBox b = c->read_box();
         for (int i = 0; i < n; i++) {
             if (pos) { pos[3 * (size_t)i] = hb[i].x; pos[3 * (size_t)i + 1] = hb[i].y; pos[3 * (size_t)i + 2] = hb[i].z; }
@@ -637,6 +640,25 @@ int sm_mc_stats(sm_ctx *c, int64_t *trials, int64_t *accepted)
     });
 }
 
+int sm_mc_hist(sm_ctx *c, int32_t cap, double *accepted, double *rejected, int32_t *n_out)
+{
+    return guarded([&] {
+        if (!c || !n_out || cap < 0) invalid("bad argument");
+        const double dL = c->P.delta_lxy;
+        int nb = 0;
+        if (dL != 0) nb = (int)(2.0 * dL / (dL / 101.0)) + 1;
+        if (nb > 256) nb = 256;
+        *n_out = nb;
+        if (nb == 0) return;
+        DevBox b = c->read_box();
+        const int m = nb < cap ? nb : cap;
+        for (int i = 0; i < m; i++) {
+            if (accepted) accepted[i] = b.hist_accept[i];
+            if (rejected) rejected[i] = b.hist_reject[i];
+        }
+    });
+}
+
 int sm_measure(sm_ctx *c, double out[3])
 {
     return guarded([&] {
@@ -692,7 +714,7 @@ int sm_pair_count(sm_ctx *c, int32_t *counts, int64_t *total)
     return guarded([&] {
         if (!c) invalid("null context");
         c->build(0);
-        launch_pair_gather(c->st, c->P, GATHER_COUNT, c->sbeads, c->skey(), c->sidx(), c->cstart, c->cend, c->box,
+        launch_pair_gather(c->st, c->P, c->chunk, GATHER_COUNT, c->sbeads, c->skey(), c->sidx(), c->cstart, c->cend, c->box,
                            c->uconstD, c->partA, c->dcounts, nullptr, nullptr, 0);
         c->check_launch();
         std::vector<int32_t> h(c->n);
@@ -717,7 +739,7 @@ int sm_pair_list(sm_ctx *c, int32_t *pairs, int64_t cap, int64_t *n_out)
         }
         c->build(0);
         HIPCHK(hipMemsetAsync(c->dnp, 0, sizeof(unsigned long long), c->st));
-        launch_pair_gather(c->st, c->P, GATHER_LIST, c->sbeads, c->skey(), c->sidx(), c->cstart, c->cend, c->box,
+        launch_pair_gather(c->st, c->P, c->chunk, GATHER_LIST, c->sbeads, c->skey(), c->sidx(), c->cstart, c->cend, c->box,
                            c->uconstD, c->partA, c->dcounts, c->dpairs, c->dnp, cap);
         c->check_launch();
         unsigned long long np = 0;
@@ -740,7 +762,7 @@ int sm_potential_delta(sm_ctx *c, const double scale[3], double *du_pair, double
         HIPCHK(hipMemcpyAsync(&c->box->scale, scale, 3 * sizeof(double), hipMemcpyHostToDevice, c->st));
         c->build(0);
         c->delta_device();
-        launch_reduce_sum(c->st, c->partA, gather_blocks(c->n), c->dres + 0);
+        launch_reduce_sum(c->st, c->partA, cells_blocks(c->n, c->chunk), c->dres + 0);
         launch_reduce_sum(c->st, c->partB, chain_blocks(c->nchains), c->dres + 1);
         c->check_launch();
         double r[2];
@@ -749,6 +771,22 @@ int sm_potential_delta(sm_ctx *c, const double scale[3], double *du_pair, double
         if (du_pair) *du_pair = r[0];
         if (du_chain) *du_chain = r[1];
         c->check_flags();
+    });
+}
+
+int sm_tune(sm_ctx *c, const char *knob, int32_t value)
+{
+    return guarded([&] {
+        if (!c || !knob) invalid("null argument");
+        if (std::strcmp(knob, "pair_chunk") == 0) {
+            if (value < 1 || value > 4096) invalid("pair_chunk must be in [1, 4096]");
+            c->chunk = value;
+        } else if (std::strcmp(knob, "pair_variant") == 0) {
+            if (value < 0 || value > 1) invalid("pair_variant must be 0 or 1");
+            c->pair_variant = value;
+        } else {
+            invalid(std::string("unknown knob ") + knob);
+        }
     });
 }
 

@@ -48,7 +48,7 @@ struct DevBox {
     double meas[3];
 };
 
-enum : int { FLAG_BAD_POSITION = 1, FLAG_CELL_OVERFLOW = 2 };
+enum : int { FLAG_BAD_POSITION = 1, FLAG_CELL_OVERFLOW = 2, FLAG_SORT_TIMEOUT = 4 };
 
 struct Params {
     int n;
@@ -70,9 +70,11 @@ void launch_first(hipStream_t st, const Params &P, Bead *beads, float4 *vel, con
                   const DevBox *box, uint32_t *key, uint32_t *idx, int integrate);
 void launch_reorder(hipStream_t st, int n, const uint32_t *skey, const uint32_t *sidx, const Bead *beads,
                     Bead *sbeads, int *cstart, int *cend);
-void launch_pair_force(hipStream_t st, const Params &P, int chunk, const Bead *sbeads, const uint32_t *skey,
-                       const uint32_t *sidx, const int *cstart, const int *cend, const DevBox *box,
-                       const float *fconst, float4 *fpair);
+// variant 0: one phase (FP64 test, force law under the branch); 1: FP32 screen into a
+// per-lane hit list, then FP64 test + force law on the hits
+void launch_pair_force(hipStream_t st, const Params &P, int chunk, int variant, const Bead *sbeads,
+                       const uint32_t *skey, const uint32_t *sidx, const int *cstart, const int *cend,
+                       const DevBox *box, const float *fconst, float4 *fpair);
 void launch_bonded_kick(hipStream_t st, const Params &P, const Bead *beads, const int2 *chain,
                         const double4 *cconst, const float4 *fpair, float4 *vel, float4 *acc, const DevBox *box,
                         long long step, int mode, float4 *fchain_out);
@@ -82,10 +84,11 @@ void launch_second(hipStream_t st, int n, const Bead *beads, const float4 *acc, 
 // FP64 gathers over the 27-cell stencil; results per block in partial[], reduced by
 // launch_reduce into out[0].
 enum : int { GATHER_U = 0, GATHER_DU = 1, GATHER_COUNT = 2, GATHER_LIST = 3 };
-int gather_blocks(int n);
-void launch_pair_gather(hipStream_t st, const Params &P, int mode, const Bead *sbeads, const uint32_t *skey,
-                        const uint32_t *sidx, const int *cstart, const int *cend, const DevBox *box,
-                        const double *uconst, double *partial, int *counts, int *pairs,
+int gather_blocks(int n);                 // 256-thread blocks over n items
+int cells_blocks(int n, int chunk);       // workgroups of the cell traversals (sm_pair.hip)
+void launch_pair_gather(hipStream_t st, const Params &P, int chunk, int mode, const Bead *sbeads,
+                        const uint32_t *skey, const uint32_t *sidx, const int *cstart, const int *cend,
+                        const DevBox *box, const double *uconst, double *partial, int *counts, int *pairs,
                         unsigned long long *npairs, long long cap);
 int chain_blocks(int nchains);
 void launch_chain_energy(hipStream_t st, int nchains, const int2 *chains, const double4 *cconst,
@@ -102,11 +105,12 @@ void launch_rescale(hipStream_t st, int n, Bead *beads, const DevBox *box);
 // ---- radix sort (sm_sort.hip) ---------------------------------------------------------
 struct SortBuffers {
     uint32_t *k0, *v0, *k1, *v1;
-    uint32_t *hist;      // 256 * tiles
+    uint32_t *scratch;   // sort_scratch_words(n, 4) words
+    int *errflag;        // receives FLAG_SORT_TIMEOUT
     int n;
     int tiles;
 };
-size_t sort_hist_entries(int n);
+size_t sort_scratch_words(int n, int max_passes);
 // Stable LSD sort of (k0,v0) by the low `bits` bits; returns which buffer pair holds the
 // result (0 -> k0/v0, 1 -> k1/v1).
 int radix_sort(hipStream_t st, SortBuffers &b, int bits);

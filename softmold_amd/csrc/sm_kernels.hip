@@ -104,185 +104,6 @@ __global__ __launch_bounds__(kBlock) void k_reorder(int n, const uint32_t *__res
     if (s == n - 1 || skey[s + 1] != k) cend[k] = s + 1;
 }
 
-__device__ inline int wave_incl_scan(int v, int lane)
-{
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const int t = __shfl_up(v, off);
-        if (lane >= off) v += t;
-    }
-    return v;
-}
-
-__device__ inline void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Workgroups are dispatched round-robin over the 8 XCDs; map them so each XCD gets one
-// contiguous range of the sorted particles (shared neighbour cells stay in one L2).
-__device__ inline int xcd_block(int b, int nblocks)
-{
-    const int x = b & 7, q = nblocks >> 3, r = nblocks & 7;
-    return x * q + min(x, r) + (b >> 3);
-}
-
-// ---------------------------------------------------------------------------------------
-// Pair force.  One wave64 per chunk of `chunk` sorted slots; it owns the cells whose run
-// starts in the chunk.  Per cell: lanes 0..26 look up the 27 stencil cells in the dense
-// table, a wave scan lays their particles out in LDS (FP64, periodic image applied),
-// then tpp = 64/H lanes serve each of the H home particles and split the candidates;
-// partial forces are summed across the tpp lanes through LDS in lane order.
-// Position differences and the r^2 < rc^2 test are FP64 (no contraction, as MD.h:798-810
-// evaluates them); the force law is FP32 (MD.h:817-828; include/cuda/potentials.h:10-31).
-// ---------------------------------------------------------------------------------------
-constexpr int kPairWaves = 4;
-constexpr int kMaxC = 256;
-
-__global__ __launch_bounds__(kPairWaves * kWave) void k_pair_force(
-    int n, int nT, int chunk, int nblocks, double cut2, const Bead *__restrict__ sb,
-    const uint32_t *__restrict__ skey, const uint32_t *__restrict__ sidx, const int *__restrict__ cstart,
-    const int *__restrict__ cend, const DevBox *__restrict__ box, const float *__restrict__ fconst,
-    float4 *__restrict__ fout)
-{
-    __shared__ float sC[6 * kMaxTypes * kMaxTypes];
-    __shared__ double sX[kPairWaves][kMaxC];
-    __shared__ double sY[kPairWaves][kMaxC];
-    __shared__ double sZ[kPairWaves][kMaxC];
-    __shared__ int sT[kPairWaves][kMaxC];
-    __shared__ float red[kPairWaves][3][kWave];
-
-    for (int t = threadIdx.x; t < 6 * nT * nT; t += blockDim.x) sC[t] = fconst[t];
-    __syncthreads();
-
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
-    const long long a0 = ((long long)xcd_block(blockIdx.x, nblocks) * kPairWaves + wave) * chunk;
-    if (a0 >= n) return;
-    const int a = (int)a0;
-    const int aend = min(a + chunk, n);
-    const int nx = box->nc[0], ny = box->nc[1], nz = box->nc[2];
-    const double Lx = box->L[0], Ly = box->L[1], Lz = box->L[2];
-    double *X = sX[wave], *Y = sY[wave], *Z = sZ[wave];
-    int *T = sT[wave];
-
-    for (int base = a; base < aend; base += kWave) {
-        const int s = base + lane;
-        bool st = false;
-        if (s < aend) st = (s == 0) || (skey[s - 1] != skey[s]);
-        unsigned long long starts = __ballot(st);
-        while (starts) {
-            const int s0 = base + __ffsll((long long)starts) - 1;
-            starts &= starts - 1;
-            const uint32_t k = skey[s0];
-            const int nh = cend[k] - s0;
-            const int kx = (int)(k % (uint32_t)nx);
-            const int ky = (int)((k / (uint32_t)nx) % (uint32_t)ny);
-            const int kz = (int)(k / (uint32_t)(nx * ny));
-            int nst = 0, ncnt = 0;
-            double shx = 0.0, shy = 0.0, shz = 0.0;
-            if (lane < 27) {
-                int X0 = kx + lane % 3 - 1, Y0 = ky + (lane / 3) % 3 - 1, Z0 = kz + lane / 9 - 1;
-                if (X0 < 0) { X0 += nx; shx = -Lx; } else if (X0 >= nx) { X0 -= nx; shx = Lx; }
-                if (Y0 < 0) { Y0 += ny; shy = -Ly; } else if (Y0 >= ny) { Y0 -= ny; shy = Ly; }
-                if (Z0 < 0) { Z0 += nz; shz = -Lz; } else if (Z0 >= nz) { Z0 -= nz; shz = Lz; }
-                const uint32_t kk = (uint32_t)(X0 + Y0 * nx + Z0 * nx * ny);
-                const int c0 = cstart[kk];
-                if ((unsigned)c0 < (unsigned)n && skey[c0] == kk) {
-                    nst = c0;
-                    ncnt = cend[kk] - c0;
-                }
-            }
-            const int incl = wave_incl_scan(ncnt, lane);
-            const int excl = incl - ncnt;
-            const int total = __shfl(incl, 26);
-            const int selfOff = __shfl(excl, 13);
-
-            for (int h0 = 0; h0 < nh; h0 += kWave) {
-                const int H = min(kWave, nh - h0);
-                const int tpp = kWave / H;
-                const int hi = lane / tpp;
-                const int sub = lane - hi * tpp;
-                const bool act = hi < H;
-                double hx = 0.0, hy = 0.0, hz = 0.0;
-                int ht = 0;
-                if (act) {
-                    const Bead hb = sb[s0 + h0 + hi];
-                    hx = hb.x; hy = hb.y; hz = hb.z; ht = hb.type;
-                }
-                const int self = selfOff + h0 + hi;
-                const float *Crow = sC + 6 * ht * nT;
-                float fx = 0.f, fy = 0.f, fz = 0.f;
-                for (int cb = 0; cb < total; cb += kMaxC) {
-                    const int ce = min(total, cb + kMaxC);
-                    for (int c = 0; c < 27; c++) {
-                        const int ccnt = __shfl(ncnt, c);
-                        if (ccnt == 0) continue;
-                        const int coff = __shfl(excl, c);
-                        const int lo = max(coff, cb), hi2 = min(coff + ccnt, ce);
-                        if (lo >= hi2) continue;
-                        const int cst = __shfl(nst, c);
-                        const double sx = __shfl(shx, c), sy = __shfl(shy, c), sz = __shfl(shz, c);
-                        for (int g = lo + lane; g < hi2; g += kWave) {
-                            const Bead q = sb[cst + (g - coff)];
-                            X[g - cb] = q.x + sx;
-                            Y[g - cb] = q.y + sy;
-                            Z[g - cb] = q.z + sz;
-                            T[g - cb] = q.type;
-                        }
-                    }
-                    wave_lds_sync();
-                    if (act) {
-                        for (int j = cb + sub; j < ce; j += tpp) {
-                            const int jj = j - cb;
-                            const double dxd = hx - X[jj];
-                            const double dyd = hy - Y[jj];
-                            const double dzd = hz - Z[jj];
-                            const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(dxd, dxd), __dmul_rn(dyd, dyd)),
-                                                        __dmul_rn(dzd, dzd));
-                            if (r2 < cut2 && j != self) {
-                                const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
-                                // r = r0 + corr: one Newton step against the exact FP64 residual
-                                // r^2 - r0^2, so r carries no FP32 rounding of r^2 into
-                                // m = C0 - r (the force near r_m is C1 * m with C1 up to ~400).
-                                const float r0 = __builtin_amdgcn_sqrtf((float)r2);
-                                float ri = __builtin_amdgcn_rcpf(r0);
-                                const float corr = (float)fma(-(double)r0, (double)r0, r2) * (0.5f * ri);
-                                const float r = r0 + corr;
-                                ri = ri * (2.0f - r * ri);
-                                const float *C = Crow + 6 * T[jj];
-                                const int off = (r >= C[0]) ? 3 : 0;   // int(r/r_m)*3, clamped
-                                const float m = (C[off] - r0) - corr;  // C[off] - r0 is exact
-                                const float mag = ((C[off + 1] - C[off + 2] * m) * m) * ri;
-                                fx += dx * mag;
-                                fy += dy * mag;
-                                fz += dz * mag;
-                            }
-                        }
-                    }
-                    wave_lds_sync();
-                }
-                red[wave][0][lane] = fx;
-                red[wave][1][lane] = fy;
-                red[wave][2][lane] = fz;
-                wave_lds_sync();
-                if (act && sub == 0) {
-                    float ax = 0.f, ay = 0.f, az = 0.f;
-                    for (int t = 0; t < tpp; t++) {
-                        ax += red[wave][0][lane + t];
-                        ay += red[wave][1][lane + t];
-                        az += red[wave][2][lane + t];
-                    }
-                    fout[sidx[s0 + h0 + hi]] = make_float4(ax, ay, az, 0.f);
-                }
-                wave_lds_sync();
-            }
-        }
-    }
-}
-
 // ---------------------------------------------------------------------------------------
 // CHAIN bonded forces by bead role (bond: harmonicF MD.h:384-403; bend: bendF
 // MD.h:683-723; chain walk system.h:1782-1866), fused with the Langevin force
@@ -442,126 +263,6 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const double *__restrict__ pa
     __shared__ double lds[kBlock];
     const double s = reduce_partials(partial, count, lds);
     if (threadIdx.x == 0) *out = s;
-}
-
-// Potential<double> (MD.h:895-930)
-__device__ inline double pair_u(double r2, const double *C)
-{
-    const double dr = sqrt(r2);
-    double pot;
-    if (dr <= C[0]) {
-        pot = C[0] - dr;
-        pot = C[1] * pot * pot + C[2];
-    } else {
-        pot = C[3] - dr;
-        pot = pot * pot * (C[4] - pot * C[5]);
-    }
-    return pot;
-}
-
-__device__ inline double sq3(double x, double y, double z)
-{
-    return __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z));
-}
-
-// Forward half shell of the reference (cellOpt.h:715) as a 27-bit mask over stencil index
-// (dx+1) + 3(dy+1) + 9(dz+1); the pair (home cell, stencil cell) is owned by the home
-// cell iff the offset is in the mask.
-__device__ constexpr uint32_t forward_mask()
-{
-    const int nb[13][3] = {{1, 0, 0}, {0, 1, 0}, {1, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1},
-                           {-1, 0, 1}, {0, -1, 1}, {-1, 1, 1}, {1, -1, 1}, {-1, -1, 1}, {1, -1, 0}};
-    uint32_t m = 0;
-    for (int j = 0; j < 13; j++) m |= 1u << ((nb[j][0] + 1) + 3 * (nb[j][1] + 1) + 9 * (nb[j][2] + 1));
-    return m;
-}
-
-// One thread per sorted particle, FP64 throughout, differences taken in the direction the
-// reference evaluates each pair (owner cell's particle minus the imaged neighbour):
-//   GATHER_U     sum_j U(r_ij)/2                                 (computePotential)
-//   GATHER_DU    sum_j [U(d_ij) - U(scale o d_ij)]/2 over the stencil (computeDPotential)
-//   GATHER_COUNT neighbours with r^2 < rc^2                        (FORCE_COUNT)
-//   GATHER_LIST  pairs (i<j) with r^2 < rc^2
-__global__ __launch_bounds__(kBlock) void k_gather(int mode, int n, int nT, double cut2,
-                                                   const Bead *__restrict__ sb, const uint32_t *__restrict__ skey,
-                                                   const uint32_t *__restrict__ sidx, const int *__restrict__ cstart,
-                                                   const int *__restrict__ cend, const DevBox *__restrict__ box,
-                                                   const double *__restrict__ uconst, double *__restrict__ partial,
-                                                   int *__restrict__ counts, int *__restrict__ pairs,
-                                                   unsigned long long *__restrict__ npairs, long long cap)
-{
-    __shared__ double sU[6 * kMaxTypes * kMaxTypes];
-    __shared__ double lds[kBlock];
-    for (int t = threadIdx.x; t < 6 * nT * nT; t += kBlock) sU[t] = uconst[t];
-    __syncthreads();
-    const int s = blockIdx.x * kBlock + threadIdx.x;
-    double accu = 0.0;
-    if (s < n) {
-        const uint32_t fwd = forward_mask();
-        const int nx = box->nc[0], ny = box->nc[1], nz = box->nc[2];
-        const double Lx = box->L[0], Ly = box->L[1], Lz = box->L[2];
-        const double ax = box->scale[0], ay = box->scale[1], az = box->scale[2];
-        const Bead h = sb[s];
-        const uint32_t me = sidx[s];
-        const uint32_t k = skey[s];
-        const int kx = (int)(k % (uint32_t)nx);
-        const int ky = (int)((k / (uint32_t)nx) % (uint32_t)ny);
-        const int kz = (int)(k / (uint32_t)(nx * ny));
-        int cnt = 0;
-        for (int c = 0; c < 27; c++) {
-            int X0 = kx + c % 3 - 1, Y0 = ky + (c / 3) % 3 - 1, Z0 = kz + c / 9 - 1;
-            double sx = 0.0, sy = 0.0, sz = 0.0;
-            if (X0 < 0) { X0 += nx; sx = -Lx; } else if (X0 >= nx) { X0 -= nx; sx = Lx; }
-            if (Y0 < 0) { Y0 += ny; sy = -Ly; } else if (Y0 >= ny) { Y0 -= ny; sy = Ly; }
-            if (Z0 < 0) { Z0 += nz; sz = -Lz; } else if (Z0 >= nz) { Z0 -= nz; sz = Lz; }
-            const uint32_t kk = (uint32_t)(X0 + Y0 * nx + Z0 * nx * ny);
-            const int c0 = cstart[kk];
-            if (!((unsigned)c0 < (unsigned)n && skey[c0] == kk)) continue;
-            const int c1 = cend[kk];
-            const bool own = (c == 13) || ((fwd >> c) & 1u);
-            for (int q = c0; q < c1; q++) {
-                if (q == s) continue;
-                const Bead o = sb[q];
-                // owner's particle minus the other, image shift on the non-owner
-                double dx, dy, dz;
-                double ox, oy, oz, px, py, pz;  // owner-side, other-side positions (imaged)
-                if (own) {
-                    ox = h.x; oy = h.y; oz = h.z;
-                    px = o.x + sx; py = o.y + sy; pz = o.z + sz;
-                } else {
-                    ox = o.x; oy = o.y; oz = o.z;
-                    px = h.x - sx; py = h.y - sy; pz = h.z - sz;
-                }
-                dx = ox - px; dy = oy - py; dz = oz - pz;
-                const double r2 = sq3(dx, dy, dz);
-                const int t1 = own ? h.type : o.type, t2 = own ? o.type : h.type;
-                const double *C = sU + 6 * (t1 * nT + t2);
-                if (mode == GATHER_U) {
-                    if (r2 < cut2) accu += 0.5 * pair_u(r2, C);
-                } else if (mode == GATHER_DU) {
-                    const double oldP = (r2 < cut2) ? pair_u(r2, C) : 0.0;
-                    const double ex = ox * ax - px * ax, ey = oy * ay - py * ay, ez = oz * az - pz * az;
-                    const double r2n = sq3(ex, ey, ez);
-                    const double newP = (r2n < cut2) ? pair_u(r2n, C) : 0.0;
-                    accu += 0.5 * (oldP - newP);
-                } else if (r2 < cut2) {
-                    cnt++;
-                    if (mode == GATHER_LIST && me < sidx[q]) {
-                        const unsigned long long slot = atomicAdd(npairs, 1ull);
-                        if ((long long)slot < cap) {
-                            pairs[2 * slot] = (int)me;
-                            pairs[2 * slot + 1] = (int)sidx[q];
-                        }
-                    }
-                }
-            }
-        }
-        if (mode == GATHER_COUNT) counts[me] = cnt;
-    }
-    if (mode == GATHER_U || mode == GATHER_DU) {
-        const double b = block_sum(accu, lds);
-        if (threadIdx.x == 0) partial[blockIdx.x] = b;
-    }
 }
 
 // CHAIN energies in FP64 (harmonicP MD.h:422-431, bendP MD.h:769-789, walk
@@ -743,16 +444,6 @@ void launch_reorder(hipStream_t st, int n, const uint32_t *skey, const uint32_t 
     hipLaunchKernelGGL(k_reorder, dim3(blocks(n)), dim3(kBlock), 0, st, n, skey, sidx, beads, sbeads, cstart, cend);
 }
 
-void launch_pair_force(hipStream_t st, const Params &P, int chunk, const Bead *sbeads, const uint32_t *skey,
-                       const uint32_t *sidx, const int *cstart, const int *cend, const DevBox *box,
-                       const float *fconst, float4 *fpair)
-{
-    const long long waves = ((long long)P.n + chunk - 1) / chunk;
-    const int nb = (int)((waves + kPairWaves - 1) / kPairWaves);
-    hipLaunchKernelGGL(k_pair_force, dim3(nb), dim3(kPairWaves * kWave), 0, st, P.n, P.nT, chunk, nb,
-                       P.cutoff * P.cutoff, sbeads, skey, sidx, cstart, cend, box, fconst, fpair);
-}
-
 void launch_bonded_kick(hipStream_t st, const Params &P, const Bead *beads, const int2 *chain, const double4 *cconst,
                         const float4 *fpair, float4 *vel, float4 *acc, const DevBox *box, long long step, int mode,
                         float4 *fchain_out)
@@ -768,16 +459,6 @@ void launch_second(hipStream_t st, int n, const Bead *beads, const float4 *acc, 
 }
 
 int gather_blocks(int n) { return blocks(n > 0 ? n : 1); }
-
-void launch_pair_gather(hipStream_t st, const Params &P, int mode, const Bead *sbeads, const uint32_t *skey,
-                        const uint32_t *sidx, const int *cstart, const int *cend, const DevBox *box,
-                        const double *uconst, double *partial, int *counts, int *pairs, unsigned long long *npairs,
-                        long long cap)
-{
-    hipLaunchKernelGGL(k_gather, dim3(gather_blocks(P.n)), dim3(kBlock), 0, st, mode, P.n, P.nT,
-                       P.cutoff * P.cutoff, sbeads, skey, sidx, cstart, cend, box, uconst, partial, counts, pairs,
-                       npairs, cap);
-}
 
 int chain_blocks(int nchains) { return blocks(nchains > 0 ? nchains : 1); }
 

@@ -1,0 +1,458 @@
+// Cell-stencil traversals: the FP32 Laradji-Revalee pair force (the hot kernel) and the
+// FP64 gathers (potential, dU of a box trial, neighbour counts, pair list).
+//
+// Reference: CellOpt::computeForce / computePotential / computeDPotential
+// (include/algorithms/cellOpt.h:712-1180) with Force/Potential (include/MD.h:795-930);
+// GPU twins collideForce/Potential/DPotential_kernel (include/cuda/cell.h:564-805).
+//
+// Work decomposition: the particles are sorted by cell key; one wave64 takes a chunk of
+// `chunk` sorted slots and owns every cell whose run starts inside it.  Per cell:
+//   1. lanes 0..26 look up the 27 stencil cells in the dense cell table (validated
+//      against the sorted keys, so the table is never cleared);
+//   2. a wave scan of the 27 counts lays the candidates out contiguously and every lane
+//      stages its share into LDS (the stencil cell of an item is found by comparing
+//      against the 26 uniform offsets held in SGPRs), so all global loads of the stencil
+//      are in flight together;
+//   3. tpp = 64/H lanes serve each of the H home particles, each lane walking every
+//      tpp-th candidate; partial results meet in LDS and are combined in lane order.
+// Force: candidates are staged twice -- FP64 imaged positions, and FP32 positions
+// relative to the home cell centre.  A lane first screens kList candidates with the FP32
+// copy against a cutoff widened by more than the FP32 error, keeping the hits in a
+// per-lane LDS list; then, on the hits only, it takes the FP64 difference, applies the
+// exact FP64 test r^2 < rc^2 (no contraction, as MD.h:798-810) and evaluates the force
+// law in FP32 with a Newton-corrected r.  Gathers: FP64 throughout, each pair's
+// difference taken in the reference's direction -- the particle of the cell that owns
+// the pair in the 13-cell half shell (cellOpt.h:715) minus the imaged other.
+#include "sm_internal.hpp"
+
+namespace smdev {
+
+namespace {
+
+constexpr int kWaves = 4;
+constexpr int kList = 32;      // candidates a lane screens before evaluating its hits
+
+enum : int { M_FORCE = 0, M_U = 1, M_DU = 2, M_COUNT = 3, M_LIST = 4, M_FORCE1 = 5 };
+
+struct __align__(16) Cand {
+    double x, y, z;
+    int t, c;
+};
+
+__device__ inline int wave_incl_scan(int v, int lane)
+{
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int t = __shfl_up(v, off);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+__device__ inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; map them so each XCD gets one
+// contiguous range of the sorted particles (shared stencil cells stay in one L2).
+__device__ inline int xcd_block(int b, int nblocks)
+{
+    const int x = b & 7, q = nblocks >> 3, r = nblocks & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
+__device__ inline double sq3(double x, double y, double z)
+{
+    return __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z));
+}
+
+// Potential<double> (MD.h:895-930)
+__device__ inline double pair_u(double r2, const double *C)
+{
+    const double dr = sqrt(r2);
+    double pot;
+    if (dr <= C[0]) {
+        pot = C[0] - dr;
+        pot = C[1] * pot * pot + C[2];
+    } else {
+        pot = C[3] - dr;
+        pot = pot * pot * (C[4] - pot * C[5]);
+    }
+    return pot;
+}
+
+// Forward half shell of the reference (cellOpt.h:715) over stencil index
+// (dx+1) + 3(dy+1) + 9(dz+1).
+__device__ constexpr uint32_t forward_mask()
+{
+    const int nb[13][3] = {{1, 0, 0}, {0, 1, 0}, {1, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1},
+                           {-1, 0, 1}, {0, -1, 1}, {-1, 1, 1}, {1, -1, 1}, {-1, -1, 1}, {1, -1, 0}};
+    uint32_t m = 0;
+    for (int j = 0; j < 13; j++) m |= 1u << ((nb[j][0] + 1) + 3 * (nb[j][1] + 1) + 9 * (nb[j][2] + 1));
+    return m;
+}
+
+struct Out {
+    float4 *force;            // M_FORCE, original order
+    double *partial;          // M_U, M_DU: one FP64 sum per workgroup
+    int *counts;              // M_COUNT, original order
+    int *pairs;               // M_LIST
+    unsigned long long *npairs;
+    long long cap;
+};
+
+template <int MODE>
+__global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(4))) void k_cells(
+    int n, int nT, int chunk, int nblocks, double cut2, float cut2pre, const Bead *__restrict__ sb,
+    const uint32_t *__restrict__ skey, const uint32_t *__restrict__ sidx, const int *__restrict__ cstart,
+    const int *__restrict__ cend, const DevBox *__restrict__ box, const float *__restrict__ fconst,
+    const double *__restrict__ uconst, Out out)
+{
+    constexpr bool kF = MODE == M_FORCE || MODE == M_FORCE1;   // force (two-phase / one-phase)
+    constexpr bool k2P = MODE == M_FORCE;
+    constexpr int kMaxC = k2P ? 128 : 256;   // staged candidates per pass
+    constexpr int kStageIters = kMaxC / kWave;
+    __shared__ float sCf[kF ? 6 * kMaxTypes * kMaxTypes : 1];
+    __shared__ double sCd[kF ? 1 : 6 * kMaxTypes * kMaxTypes];
+    __shared__ Cand sCand[kWaves][kMaxC];
+    __shared__ float4 sRel[k2P ? kWaves : 1][k2P ? kMaxC : 1];
+    __shared__ uint8_t sList[k2P ? kWaves : 1][k2P ? kList : 1][kWave];   // per-lane hit lists, lane-minor
+    __shared__ float redf[kF ? kWaves : 1][3][kWave];
+    __shared__ double redd[kWaves * kWave];
+    __shared__ int redi[kWaves][kWave];
+
+    if (kF) {
+        for (int t = threadIdx.x; t < 6 * nT * nT; t += blockDim.x) sCf[t] = fconst[t];
+    } else {
+        for (int t = threadIdx.x; t < 6 * nT * nT; t += blockDim.x) sCd[t] = uconst[t];
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const long long a0 = ((long long)xcd_block(blockIdx.x, nblocks) * kWaves + wave) * chunk;
+    const int a = (int)min(a0, (long long)n);
+    const int aend = min(a + chunk, n);
+    const int nx = box->nc[0], ny = box->nc[1], nz = box->nc[2];
+    const double Lx = box->L[0], Ly = box->L[1], Lz = box->L[2];
+    const double csx = box->cs[0], csy = box->cs[1], csz = box->cs[2];
+    const double ax = box->scale[0], ay = box->scale[1], az = box->scale[2];
+    constexpr uint32_t fwd = forward_mask();
+    Cand *Cw = sCand[wave];
+    float4 *Rw = sRel[k2P ? wave : 0];
+    double accu = 0.0;
+
+    for (int base = a; base < aend; base += kWave) {
+        const int s = base + lane;
+        bool st = false;
+        if (s < aend) st = (s == 0) || (skey[s - 1] != skey[s]);
+        unsigned long long starts = __ballot(st);
+        while (starts) {
+            const int s0 = base + __ffsll((long long)starts) - 1;
+            starts &= starts - 1;
+            const uint32_t k = skey[s0];
+            const int e0 = cend[k];
+            const int kx = (int)(k % (uint32_t)nx);
+            const int ky = (int)((k / (uint32_t)nx) % (uint32_t)ny);
+            const int kz = (int)(k / (uint32_t)(nx * ny));
+            // home cell centre: FP32 positions relative to it stay within 1.5 cells
+            const double ox0 = (kx + 0.5) * csx, oy0 = (ky + 0.5) * csy, oz0 = (kz + 0.5) * csz;
+
+            // 1. stencil lookups, one cell per lane
+            int nst = 0, ncnt = 0;
+            if (lane < 27) {
+                int X0 = kx + lane % 3 - 1, Y0 = ky + (lane / 3) % 3 - 1, Z0 = kz + lane / 9 - 1;
+                X0 = X0 < 0 ? X0 + nx : (X0 >= nx ? X0 - nx : X0);
+                Y0 = Y0 < 0 ? Y0 + ny : (Y0 >= ny ? Y0 - ny : Y0);
+                Z0 = Z0 < 0 ? Z0 + nz : (Z0 >= nz ? Z0 - nz : Z0);
+                const uint32_t kk = (uint32_t)(X0 + Y0 * nx + Z0 * nx * ny);
+                const int c0 = cstart[kk];
+                const int c1 = cend[kk];
+                if ((unsigned)c0 < (unsigned)n && skey[c0] == kk) {
+                    nst = c0;
+                    ncnt = c1 - c0;
+                }
+            }
+            // 2. layout of the candidates
+            const int incl = wave_incl_scan(ncnt, lane);
+            const int excl = incl - ncnt;
+            const int total = __builtin_amdgcn_readlane(incl, 26);
+            const int selfOff = __builtin_amdgcn_readlane(excl, 13);
+            int off[26];
+#pragma unroll
+            for (int c = 0; c < 26; c++) off[c] = __builtin_amdgcn_readlane(incl, c);
+            const int nh = e0 - s0;
+
+            for (int h0 = 0; h0 < nh; h0 += kWave) {
+                const int H = min(kWave, nh - h0);
+                const int tpp = kWave / H;
+                const int hi = lane / tpp;
+                const int sub = lane - hi * tpp;
+                const bool act = hi < H;
+                Bead hb;
+                hb.x = hb.y = hb.z = 0.0;
+                hb.type = 0;
+                if (act) hb = sb[s0 + h0 + hi];
+                const int self = selfOff + h0 + hi;
+                const uint32_t me = act ? sidx[s0 + h0 + hi] : 0u;
+                const float hfx = (float)(hb.x - ox0), hfy = (float)(hb.y - oy0), hfz = (float)(hb.z - oz0);
+                float fx = 0.f, fy = 0.f, fz = 0.f;
+                int cnt = 0;
+                for (int cb = 0; cb < total; cb += kMaxC) {
+                    const int ce = min(total, cb + kMaxC);
+                    // stage [cb, ce): addresses first, then all loads, then the LDS stores
+                    int src[kStageIters], cel[kStageIters];
+#pragma unroll
+                    for (int it = 0; it < kStageIters; it++) {
+                        const int t = cb + lane + it * kWave;
+                        int c = 0;
+#pragma unroll
+                        for (int q = 0; q < 26; q++) c += (t >= off[q]) ? 1 : 0;
+                        const int stc = __shfl(nst, c), exc = __shfl(excl, c);
+                        cel[it] = c;
+                        src[it] = (t < ce) ? stc + (t - exc) : -1;
+                    }
+                    Bead qb[kStageIters];
+#pragma unroll
+                    for (int it = 0; it < kStageIters; it++)
+                        if (src[it] >= 0) qb[it] = sb[src[it]];
+#pragma unroll
+                    for (int it = 0; it < kStageIters; it++) {
+                        if (src[it] < 0) continue;
+                        const int c = cel[it];
+                        Cand cd;
+                        if (kF) {
+                            const int X0 = kx + c % 3 - 1, Y0 = ky + (c / 3) % 3 - 1, Z0 = kz + c / 9 - 1;
+                            cd.x = qb[it].x + (X0 < 0 ? -Lx : (X0 >= nx ? Lx : 0.0));
+                            cd.y = qb[it].y + (Y0 < 0 ? -Ly : (Y0 >= ny ? Ly : 0.0));
+                            cd.z = qb[it].z + (Z0 < 0 ? -Lz : (Z0 >= nz ? Lz : 0.0));
+                            if (k2P)
+                                Rw[lane + it * kWave] = make_float4((float)(cd.x - ox0), (float)(cd.y - oy0),
+                                                                    (float)(cd.z - oz0), 0.f);
+                        } else {
+                            cd.x = qb[it].x;
+                            cd.y = qb[it].y;
+                            cd.z = qb[it].z;
+                        }
+                        cd.t = qb[it].type;
+                        cd.c = kF ? 0 : (c | ((int)src[it] << 5));
+                        Cw[lane + it * kWave] = cd;
+                    }
+                    wave_lds_sync();
+                    if (MODE == M_FORCE1 && act) {
+                        // one phase: FP64 test, force law under the branch
+                        const float *Cf = sCf + 6 * hb.type * nT;
+                        for (int j = cb + sub; j < ce; j += tpp) {
+                            const Cand q = Cw[j - cb];
+                            const double dxd = hb.x - q.x;
+                            const double dyd = hb.y - q.y;
+                            const double dzd = hb.z - q.z;
+                            const double r2 = sq3(dxd, dyd, dzd);
+                            if (r2 < cut2 && j != self) {
+                                const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
+                                const float r0 = __builtin_amdgcn_sqrtf((float)r2);
+                                float ri = __builtin_amdgcn_rcpf(r0);
+                                const float corr = (float)fma(-(double)r0, (double)r0, r2) * (0.5f * ri);
+                                const float r = r0 + corr;
+                                ri = ri * (2.0f - r * ri);
+                                const float *C = Cf + 6 * q.t;
+                                const int o = (r >= C[0]) ? 3 : 0;
+                                const float m = (C[o] - r0) - corr;
+                                const float mag = ((C[o + 1] - C[o + 2] * m) * m) * ri;
+                                fx += dx * mag;
+                                fy += dy * mag;
+                                fz += dz * mag;
+                            }
+                        }
+                    } else if (k2P) {
+                        const float *Cf = sCf + 6 * hb.type * nT;
+                        uint8_t *lst = &sList[k2P ? wave : 0][0][lane];
+                        for (int jb = cb + sub; __ballot(act && jb < ce); jb += tpp * kList) {
+                            // phase 1: FP32 screen, hits kept in the lane's list
+                            int nhit = 0;
+                            if (act) {
+#pragma unroll 2
+                                for (int kk = 0; kk < kList; kk++) {
+                                    const int j = jb + kk * tpp;
+                                    if (j < ce) {
+                                        const float4 q = Rw[j - cb];
+                                        const float dx = hfx - q.x, dy = hfy - q.y, dz = hfz - q.z;
+                                        const float r2 = dx * dx + dy * dy + dz * dz;
+                                        if (r2 < cut2pre && j != self) lst[kWave * nhit++] = (uint8_t)(j - cb);
+                                    }
+                                }
+                            }
+                            int most = nhit;
+#pragma unroll
+                            for (int o = 1; o < kWave; o <<= 1) most = max(most, __shfl_xor(most, o));
+                            // phase 2: exact FP64 test and the force law on the hits
+                            for (int kk = 0; kk < most; kk++) {
+                                if (kk < nhit) {
+                                    const Cand q = Cw[lst[kWave * kk]];
+                                    const double dxd = hb.x - q.x;
+                                    const double dyd = hb.y - q.y;
+                                    const double dzd = hb.z - q.z;
+                                    const double r2 = sq3(dxd, dyd, dzd);
+                                    if (r2 < cut2) {
+                                        const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
+                                        // r = r0 + corr: one Newton step against the exact FP64
+                                        // residual r^2 - r0^2, so m = C0 - r carries no FP32
+                                        // rounding of r^2 (near r_m the force is C1 m, C1 ~ 400).
+                                        const float r0 = __builtin_amdgcn_sqrtf((float)r2);
+                                        float ri = __builtin_amdgcn_rcpf(r0);
+                                        const float corr = (float)fma(-(double)r0, (double)r0, r2) * (0.5f * ri);
+                                        const float r = r0 + corr;
+                                        ri = ri * (2.0f - r * ri);
+                                        const float *C = Cf + 6 * q.t;
+                                        const int o = (r >= C[0]) ? 3 : 0;    // int(r/r_m)*3, clamped
+                                        const float m = (C[o] - r0) - corr;   // C[o] - r0 is exact
+                                        const float mag = ((C[o + 1] - C[o + 2] * m) * m) * ri;
+                                        fx += dx * mag;
+                                        fy += dy * mag;
+                                        fz += dz * mag;
+                                    }
+                                }
+                            }
+                        }
+                    } else if (!kF && act) {
+                        for (int j = cb + sub; j < ce; j += tpp) {
+                            if (j == self) continue;
+                            const Cand q = Cw[j - cb];
+                            const int c = q.c & 31;
+                            const int X0 = kx + c % 3 - 1, Y0 = ky + (c / 3) % 3 - 1, Z0 = kz + c / 9 - 1;
+                            const double sx = X0 < 0 ? -Lx : (X0 >= nx ? Lx : 0.0);
+                            const double sy = Y0 < 0 ? -Ly : (Y0 >= ny ? Ly : 0.0);
+                            const double sz = Z0 < 0 ? -Lz : (Z0 >= nz ? Lz : 0.0);
+                            const bool own = (c == 13) || ((fwd >> c) & 1u);
+                            double ox, oy, oz, px, py, pz;
+                            if (own) {
+                                ox = hb.x; oy = hb.y; oz = hb.z;
+                                px = q.x + sx; py = q.y + sy; pz = q.z + sz;
+                            } else {
+                                ox = q.x; oy = q.y; oz = q.z;
+                                px = hb.x - sx; py = hb.y - sy; pz = hb.z - sz;
+                            }
+                            const double r2 = sq3(ox - px, oy - py, oz - pz);
+                            if (MODE == M_U || MODE == M_DU) {
+                                const double *C = sCd + 6 * ((own ? hb.type : q.t) * nT + (own ? q.t : hb.type));
+                                if (MODE == M_U) {
+                                    if (r2 < cut2) accu += 0.5 * pair_u(r2, C);
+                                } else {
+                                    const double oldP = (r2 < cut2) ? pair_u(r2, C) : 0.0;
+                                    const double r2n = sq3(ox * ax - px * ax, oy * ay - py * ay, oz * az - pz * az);
+                                    const double newP = (r2n < cut2) ? pair_u(r2n, C) : 0.0;
+                                    accu += 0.5 * (oldP - newP);
+                                }
+                            } else if (r2 < cut2) {
+                                cnt++;
+                                if (MODE == M_LIST) {
+                                    const uint32_t other = sidx[q.c >> 5];
+                                    if (me < other) {
+                                        const unsigned long long slot = atomicAdd(out.npairs, 1ull);
+                                        if ((long long)slot < out.cap) {
+                                            out.pairs[2 * slot] = (int)me;
+                                            out.pairs[2 * slot + 1] = (int)other;
+                                        }
+                                    }
+                                }
+                            }
+                        }
+                    }
+                    wave_lds_sync();
+                }
+                if (kF) {
+                    redf[kF ? wave : 0][0][lane] = fx;
+                    redf[kF ? wave : 0][1][lane] = fy;
+                    redf[kF ? wave : 0][2][lane] = fz;
+                    wave_lds_sync();
+                    if (act && sub == 0) {
+                        float sx = 0.f, sy = 0.f, sz = 0.f;
+                        for (int t = 0; t < tpp; t++) {
+                            sx += redf[kF ? wave : 0][0][lane + t];
+                            sy += redf[kF ? wave : 0][1][lane + t];
+                            sz += redf[kF ? wave : 0][2][lane + t];
+                        }
+                        out.force[me] = make_float4(sx, sy, sz, 0.f);
+                    }
+                    wave_lds_sync();
+                } else if (MODE == M_COUNT) {
+                    redi[wave][lane] = cnt;
+                    wave_lds_sync();
+                    if (act && sub == 0) {
+                        int c = 0;
+                        for (int t = 0; t < tpp; t++) c += redi[wave][lane + t];
+                        out.counts[me] = c;
+                    }
+                    wave_lds_sync();
+                }
+            }
+        }
+    }
+    if (MODE == M_U || MODE == M_DU) {
+        // fixed-shape tree over the workgroup: the result does not depend on scheduling
+        redd[threadIdx.x] = accu;
+        __syncthreads();
+        for (int o = kWaves * kWave / 2; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) redd[threadIdx.x] += redd[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out.partial[blockIdx.x] = redd[0];
+    }
+}
+
+template <int MODE>
+void launch_cells(hipStream_t st, const Params &P, int chunk, const Bead *sbeads, const uint32_t *skey,
+                  const uint32_t *sidx, const int *cstart, const int *cend, const DevBox *box,
+                  const float *fconst, const double *uconst, const Out &out)
+{
+    const int nb = cells_blocks(P.n, chunk);
+    const double cut2 = P.cutoff * P.cutoff;
+    // FP32 screen: positions relative to the home cell centre are within 1.5 cells, so
+    // each coordinate carries at most 2^-23 * 1.5 * csmax rounding; widen r^2 by far more.
+    // With at least 3 cells per axis, a cell is shorter than 4/3 of the cutoff.
+    const double csmax = P.cutoff * 4.0 / 3.0;
+    const double slack = 64.0 * P.cutoff * 1.5 * csmax * (1.0 / 8388608.0) + 1e-6 * cut2;
+    const float cut2pre = (float)(cut2 + slack);
+    hipLaunchKernelGGL(k_cells<MODE>, dim3(nb), dim3(kWaves * kWave), 0, st, P.n, P.nT, chunk, nb, cut2, cut2pre,
+                       sbeads, skey, sidx, cstart, cend, box, fconst, uconst, out);
+}
+
+}  // namespace
+
+int cells_blocks(int n, int chunk)
+{
+    const long long waves = ((long long)(n > 0 ? n : 1) + chunk - 1) / chunk;
+    return (int)((waves + kWaves - 1) / kWaves);
+}
+
+void launch_pair_force(hipStream_t st, const Params &P, int chunk, int variant, const Bead *sbeads,
+                       const uint32_t *skey, const uint32_t *sidx, const int *cstart, const int *cend,
+                       const DevBox *box, const float *fconst, float4 *fpair)
+{
+    Out o{fpair, nullptr, nullptr, nullptr, nullptr, 0};
+    if (variant == 1)
+        launch_cells<M_FORCE>(st, P, chunk, sbeads, skey, sidx, cstart, cend, box, fconst, nullptr, o);
+    else
+        launch_cells<M_FORCE1>(st, P, chunk, sbeads, skey, sidx, cstart, cend, box, fconst, nullptr, o);
+}
+
+void launch_pair_gather(hipStream_t st, const Params &P, int chunk, int mode, const Bead *sbeads,
+                        const uint32_t *skey, const uint32_t *sidx, const int *cstart, const int *cend,
+                        const DevBox *box, const double *uconst, double *partial, int *counts, int *pairs,
+                        unsigned long long *npairs, long long cap)
+{
+    Out o{nullptr, partial, counts, pairs, npairs, cap};
+    switch (mode) {
+    case GATHER_U: launch_cells<M_U>(st, P, chunk, sbeads, skey, sidx, cstart, cend, box, nullptr, uconst, o); break;
+    case GATHER_DU: launch_cells<M_DU>(st, P, chunk, sbeads, skey, sidx, cstart, cend, box, nullptr, uconst, o); break;
+    case GATHER_COUNT:
+        launch_cells<M_COUNT>(st, P, chunk, sbeads, skey, sidx, cstart, cend, box, nullptr, uconst, o);
+        break;
+    default: launch_cells<M_LIST>(st, P, chunk, sbeads, skey, sidx, cstart, cend, box, nullptr, uconst, o); break;
+    }
+}
+
+}  // namespace smdev

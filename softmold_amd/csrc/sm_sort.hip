@@ -2,13 +2,18 @@
 //
 // The reference sorts with thrust::sort_by_key (include/cuda/cell.h:254-260) and its CPU
 // path keeps unsorted linked lists (include/algorithms/cellOpt.h:572-585).  Here: 8-bit
-// digits, ceil(bits/8) passes, three kernels per pass:
-//   hist     per-tile digit counts in LDS                              (reads 4 B/key)
-//   scan     exclusive scan over the digit-major [256 x tiles] table   (one workgroup)
-//   scatter  per-tile stable ranking with wave64 ballots, then scatter (reads/writes 8 B/key)
-// A tile is 4096 items = 16 rounds of 256; within a round the 4 waves of the workgroup
-// rank their 64 items by matching the 8 digit bits with ballots, and the per-wave counts
-// go through LDS so the result is stable (equal keys keep their input order).
+// digits, ceil(bits/8) passes, one kernel per pass ("onesweep"):
+//   clear    zero the digit histograms, tile counters and look-back states
+//   ghist    digit histograms of every pass at once (order independent)
+//   pass     a workgroup takes the next tile id from a counter (so every lower tile is
+//            already running), ranks its items with wave64 ballots, publishes its
+//            per-digit counts, accumulates the counts of the tiles before it by
+//            decoupled look-back, and scatters.
+// Counting is wave-aggregated: the lanes holding one digit are found with 8 ballots and
+// their lowest lane writes the count once, so a digit shared by a whole wave (the high
+// digits of a compact system) costs one LDS write instead of 64 contended atomics.
+// The look-back spin is bounded: on timeout the kernel records FLAG_SORT_TIMEOUT in the
+// caller's flag word, clamps every write in bounds, and the host reports an error.
 #include "sm_internal.hpp"
 
 namespace smdev {
@@ -16,126 +21,185 @@ namespace smdev {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kRounds = 16;
+constexpr int kWavesPerBlock = kThreads / kWave;
+constexpr int kRounds = 4;
 constexpr int kTile = kThreads * kRounds;
-constexpr int kScanThreads = 1024;
+constexpr uint32_t kFlagAgg = 1u << 30, kFlagIncl = 2u << 30, kCountMask = (1u << 30) - 1;
+constexpr int kSpinLimit = 1 << 20;
 
-__global__ __launch_bounds__(kThreads) void k_hist(int n, const uint32_t *__restrict__ keys, int shift,
-                                                   int tiles, uint32_t *__restrict__ hist)
+// lanes of the wave holding the same 8-bit digit as this lane (invalid lanes excluded)
+__device__ inline unsigned long long match_digit(uint32_t d, bool valid)
 {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    const int tile = blockIdx.x;
-    const int base = tile * kTile;
-    for (int r = 0; r < kRounds; r++) {
-        int i = base + r * kThreads + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    hist[(size_t)threadIdx.x * tiles + tile] = h[threadIdx.x];
-}
-
-// Exclusive scan, in place, of m entries with one workgroup: each thread owns a contiguous
-// segment, segment totals are scanned in LDS, segments are rewritten.
-__global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t *__restrict__ a, int m)
-{
-    __shared__ uint32_t s[kScanThreads];
-    const int t = threadIdx.x;
-    const int seg = (m + kScanThreads - 1) / kScanThreads;
-    const int lo = t * seg;
-    const int hi = min(lo + seg, m);
-    uint32_t sum = 0;
-    for (int i = lo; i < hi; i++) sum += a[i];
-    s[t] = sum;
-    __syncthreads();
-    for (int off = 1; off < kScanThreads; off <<= 1) {
-        uint32_t v = (t >= off) ? s[t - off] : 0u;
-        __syncthreads();
-        s[t] += v;
-        __syncthreads();
-    }
-    uint32_t run = s[t] - sum;
-    for (int i = lo; i < hi; i++) {
-        uint32_t x = a[i];
-        a[i] = run;
-        run += x;
-    }
-}
-
-__global__ __launch_bounds__(kThreads) void k_scatter(int n, const uint32_t *__restrict__ kin,
-                                                      const uint32_t *__restrict__ vin,
-                                                      uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
-                                                      int shift, int tiles, const uint32_t *__restrict__ hist)
-{
-    __shared__ uint32_t base[256];
-    __shared__ uint32_t run[256];
-    __shared__ uint32_t wcnt[kThreads / kWave][256];
-    const int t = threadIdx.x;
-    const int lane = t & (kWave - 1);
-    const int wave = t / kWave;
-    const int tile = blockIdx.x;
-    base[t] = hist[(size_t)t * tiles + tile];
-    run[t] = 0;
-    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int r = 0; r < kRounds; r++) {
-        const int i = tile * kTile + r * kThreads + t;
-        const bool valid = i < n;
-        uint32_t k = 0, v = 0, d = 0;
-        if (valid) {
-            k = kin[i];
-            v = vin[i];
-            d = (k >> shift) & 255u;
-        }
-        for (int w = 0; w < kThreads / kWave; w++) wcnt[w][t] = 0;
-        __syncthreads();
-        unsigned long long peers = __ballot(valid);
+    unsigned long long peers = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
-            const bool bit = (d >> b) & 1u;
-            const unsigned long long m = __ballot(bit);
-            peers &= bit ? m : ~m;
+    for (int b = 0; b < 8; b++) {
+        const bool bit = (d >> b) & 1u;
+        const unsigned long long m = __ballot(bit);
+        peers &= bit ? m : ~m;
+    }
+    return peers;
+}
+
+__device__ inline unsigned long long lanes_below()
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+}
+
+__global__ __launch_bounds__(256) void k_clear(uint32_t *__restrict__ a, size_t count)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < count) a[i] = 0u;
+}
+
+__global__ __launch_bounds__(kThreads) void k_ghist(int n, const uint32_t *__restrict__ keys, int passes,
+                                                    uint32_t *__restrict__ ghist)
+{
+    __shared__ uint32_t h[4][256];
+    for (int p = 0; p < 4; p++) h[p][threadIdx.x] = 0;
+    __syncthreads();
+    const unsigned long long lt = lanes_below();
+    const int stride = gridDim.x * kThreads;
+    const int nloop = (n + stride - 1) / stride;   // uniform trip count: ballots need every lane
+    for (int it = 0; it < nloop; it++) {
+        const int i = it * stride + blockIdx.x * kThreads + threadIdx.x;
+        const bool valid = i < n;
+        const uint32_t k = valid ? keys[i] : 0u;
+        for (int p = 0; p < passes; p++) {
+            const uint32_t d = (k >> (8 * p)) & 255u;
+            const unsigned long long peers = match_digit(d, valid);
+            if (valid && (peers & lt) == 0ull) atomicAdd(&h[p][d], (uint32_t)__popcll(peers));
         }
-        const int rank = __popcll(peers & lt);
-        if (valid && rank == 0) wcnt[wave][d] = __popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = base[d] + run[d] + (uint32_t)rank;
-            for (int w = 0; w < wave; w++) pos += wcnt[w][d];
-            kout[pos] = k;
-            vout[pos] = v;
+    }
+    __syncthreads();
+    for (int p = 0; p < passes; p++)
+        if (h[p][threadIdx.x]) atomicAdd(&ghist[p * 256 + threadIdx.x], h[p][threadIdx.x]);
+}
+
+__global__ __launch_bounds__(kThreads) void k_onesweep(int n, const uint32_t *__restrict__ kin,
+                                                       const uint32_t *__restrict__ vin,
+                                                       uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                                                       int shift, const uint32_t *__restrict__ ghist,
+                                                       uint32_t *__restrict__ status, uint32_t *__restrict__ tilectr,
+                                                       int *__restrict__ errflag)
+{
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t scan[256];
+    // per (round, wave, digit): item count, then its exclusive output position
+    __shared__ uint32_t wc[kRounds][kWavesPerBlock][256];
+    const int t = threadIdx.x;
+    const int wave = t / kWave;
+    if (t == 0) s_tile = atomicAdd(tilectr, 1u);
+    for (int r = 0; r < kRounds; r++)
+        for (int w = 0; w < kWavesPerBlock; w++) wc[r][w][t] = 0u;
+    __syncthreads();
+    const int tile = (int)s_tile;
+    const int base = tile * kTile;
+    const unsigned long long lt = lanes_below();
+
+    uint32_t kr[kRounds], vr[kRounds], dr[kRounds], rk[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; r++) {
+        const int i = base + r * kThreads + t;
+        const bool valid = i < n;
+        kr[r] = valid ? kin[i] : 0u;
+        vr[r] = valid ? vin[i] : 0u;
+        dr[r] = valid ? (kr[r] >> shift) & 255u : 256u;   // 256: no item
+        const unsigned long long peers = match_digit(dr[r] & 255u, valid);
+        rk[r] = (uint32_t)__popcll(peers & lt);
+        if (valid && rk[r] == 0u) wc[r][wave][dr[r]] = (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+
+    // tile count of digit t, published for the tiles after this one
+    uint32_t mine = 0;
+    for (int r = 0; r < kRounds; r++)
+        for (int w = 0; w < kWavesPerBlock; w++) mine += wc[r][w][t];
+    uint32_t *st = status + (size_t)tile * 256 + t;
+    __hip_atomic_store(st, (tile == 0 ? kFlagIncl : kFlagAgg) | mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t prefix = 0;
+    for (int pt = tile - 1; pt >= 0;) {
+        const uint32_t v = __hip_atomic_load(status + (size_t)pt * 256 + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t f = v & ~kCountMask;
+        if (f == 0u) {
+            int spins = 0;
+            uint32_t w = 0;
+            do {
+                __builtin_amdgcn_s_sleep(1);
+                w = __hip_atomic_load(status + (size_t)pt * 256 + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } while ((w & ~kCountMask) == 0u && ++spins < kSpinLimit);
+            if ((w & ~kCountMask) == 0u) {
+                atomicOr(errflag, FLAG_SORT_TIMEOUT);
+                break;
+            }
+            continue;
         }
+        prefix += v & kCountMask;
+        if (f == kFlagIncl) break;
+        pt--;
+    }
+    if (tile > 0)
+        __hip_atomic_store(st, kFlagIncl | (prefix + mine), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+
+    // digit base: keys of smaller digits (all tiles) + this digit in earlier tiles
+    const uint32_t g = ghist[t];
+    scan[t] = g;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t v = (t >= off) ? scan[t - off] : 0u;
         __syncthreads();
-        uint32_t add = 0;
-        for (int w = 0; w < kThreads / kWave; w++) add += wcnt[w][t];
-        run[t] += add;
+        scan[t] += v;
         __syncthreads();
+    }
+    uint32_t pos = scan[t] - g + prefix;
+    for (int r = 0; r < kRounds; r++)
+        for (int w = 0; w < kWavesPerBlock; w++) {
+            const uint32_t c = wc[r][w][t];
+            wc[r][w][t] = pos;
+            pos += c;
+        }
+    __syncthreads();
+
+#pragma unroll
+    for (int r = 0; r < kRounds; r++) {
+        if (dr[r] < 256u) {
+            uint32_t p = wc[r][wave][dr[r]] + rk[r];
+            p = p < (uint32_t)n ? p : (uint32_t)(n - 1);  // only reachable after a timeout
+            kout[p] = kr[r];
+            vout[p] = vr[r];
+        }
     }
 }
 
 }  // namespace
 
-size_t sort_hist_entries(int n)
+size_t sort_scratch_words(int n, int max_passes)
 {
-    int tiles = (n + kTile - 1) / kTile;
-    if (tiles < 1) tiles = 1;
-    return (size_t)256 * tiles;
+    const size_t tiles = (size_t)((n + kTile - 1) / kTile) + 1;
+    // ghist [4][256] + tile counters [4] + status [passes][tiles][256]
+    return 4 * 256 + 4 + (size_t)max_passes * tiles * 256;
 }
 
 int radix_sort(hipStream_t st, SortBuffers &b, int bits)
 {
     const int n = b.n;
     if (n <= 1) return 0;
-    b.tiles = (n + kTile - 1) / kTile;
     int passes = (bits + 7) / 8;
     if (passes < 1) passes = 1;
+    if (passes > 4) passes = 4;
+    const int tiles = (n + kTile - 1) / kTile;
+    b.tiles = tiles;
+    uint32_t *ghist = b.scratch;
+    uint32_t *tilectr = ghist + 4 * 256;
+    uint32_t *status = tilectr + 4;
+    const size_t words = 4 * 256 + 4 + (size_t)passes * tiles * 256;
+    hipLaunchKernelGGL(k_clear, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, st, b.scratch, words);
+    const int gb = min(512, (n + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(k_ghist, dim3(gb), dim3(kThreads), 0, st, n, b.k0, passes, ghist);
     uint32_t *ki = b.k0, *vi = b.v0, *ko = b.k1, *vo = b.v1;
     for (int p = 0; p < passes; p++) {
-        const int shift = 8 * p;
-        hipLaunchKernelGGL(k_hist, dim3(b.tiles), dim3(kThreads), 0, st, n, ki, shift, b.tiles, b.hist);
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, st, b.hist, 256 * b.tiles);
-        hipLaunchKernelGGL(k_scatter, dim3(b.tiles), dim3(kThreads), 0, st, n, ki, vi, ko, vo, shift, b.tiles,
-                           b.hist);
+        hipLaunchKernelGGL(k_onesweep, dim3(tiles), dim3(kThreads), 0, st, n, ki, vi, ko, vo, 8 * p, ghist + p * 256,
+                           status + (size_t)p * tiles * 256, tilectr + p, b.errflag);
         uint32_t *tk = ki, *tv = vi;
         ki = ko; vi = vo; ko = tk; vo = tv;
     }

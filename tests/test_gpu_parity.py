@@ -122,7 +122,9 @@ def test_energies_fp64(name):
     s = sm.Solver.from_system(d)
     U_pair, U_chain, K = s.measure()
     o = oracle_of(d)
-    assert U_pair == pytest.approx(o.pair_potential(), rel=1e-12)
+    # FP64 sums of ~10^5 terms in different orders (and the oracle's own OpenMP order
+    # varies run to run at ~1e-12): 1e-10 relative
+    assert U_pair == pytest.approx(o.pair_potential(), rel=1e-10)
     assert U_chain == pytest.approx(o.chain_potential(), rel=1e-6, abs=1e-12)
     # velocities live in FP32 on the device
     assert K == pytest.approx(o.kinetic(), rel=1e-6)
@@ -131,7 +133,7 @@ def test_energies_fp64(name):
 def test_cfg1_known_answer():
     s = sm.Solver.from_system(system("L5000"))
     U_pair, _, K = s.measure()
-    assert U_pair == pytest.approx(-388321.0139620534, rel=1e-12)
+    assert U_pair == pytest.approx(-388321.0139620534, rel=1e-10)
     assert K == pytest.approx(1.5 * 15000 * 3.0, rel=1e-6)
 
 
