@@ -1,19 +1,24 @@
 """Throughput of the MI355X MD-step solver on one GPU.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched]
     python bench.py --calib-copy      # plain 512 MiB device copies, for HBM counter scaling
 
-Workload: the 80,000-lipid vesicle (240,000 beads) from the liposome generator (seed 1234,
-areal density 3.45, box 400^3, dt 0.02, T 3, gamma 1, tension 0 so no box moves), built
-in memory with the product's generator (byte-identical to the reference generator).  A
-step is one full MD step: Verlet first half + cell key, radix sort, reorder + cell table,
-pair force, CHAIN forces + Langevin + second half-kick.  Inputs are resident in HBM
-before timing starts; no measurement or file output inside the timed loop.
+Workload (`value`): the 80,000-lipid vesicle (240,000 beads) from the liposome generator
+(seed 1234, areal density 3.45, box 400^3, dt 0.02, T 3, gamma 1, tension 0 so no box
+moves), built in memory with the product's generator (byte-identical to the reference
+generator).  A step is one full MD step: Verlet first half + cell key, radix sort,
+reorder + cell table, pair force, CHAIN forces + Langevin + second half-kick.  Inputs are
+resident in HBM before timing starts; no measurement or file output inside the timed loop.
+
+`batched`: the 64-replica tension x temperature sweep (softmold_amd.sweep) stepped as one
+batch after the autotuner picked its launch shape, with box moves every 8 steps, next to
+the same replicas run one after another through the single-system solver.
 
 Prints one JSON line: particle-steps/s, the pair kernel's roofline record (algorithmic
-bytes per launch over its HIP-event-timed average duration) and the CPU baseline (the
-reference MD binary built from source in oracle/_ref, or the FP64 oracle port when it is
-absent), timed on this host on a bounded sample of the same workload.
+bytes per launch over its HIP-event-timed average duration), the batched sweep, and the
+CPU baseline (the reference MD binary built from source in oracle/_ref, or the FP64
+oracle port when it is absent), timed on this host on a bounded sample of the plain
+workload.
 """
 from __future__ import annotations
 
@@ -112,6 +117,60 @@ def cpu_baseline(n_beads: int) -> dict:
         shutil.rmtree(work, ignore_errors=True)
 
 
+def bench_batched(steps: int, warmup: int, seq_steps: int) -> dict:
+    """The 64-replica tension x temperature sweep stepped as one batch, against the same
+    replicas run one after another through the single-system solver."""
+    import torch
+
+    import softmold_amd as sm
+    from softmold_amd.sweep import replica_sweep
+
+    systems = replica_sweep()
+    nb = sum(len(d["type"]) for d in systems)
+    b = sm.Solver.batch(systems)
+    b.init_forces()
+    tuned_ms = b.autotune(32)
+    knobs = {k: b.tune_get(k) for k in ("pair_chunk", "pair_variant", "graph")}
+    b.run(1, warmup)
+    b.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run(1 + warmup, steps)
+    b.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    trials, accepted = b.mc_stats()
+    b.close()
+    batched = nb * steps / wall
+    log(f"batched: {len(systems)} replicas, {nb} beads, {1e3 * wall / steps:.3f} ms/step, "
+        f"{batched:.4g} particle-steps/s, knobs {knobs}, MC acceptance {accepted.sum() / max(trials.sum(), 1):.3f}")
+
+    # the plain solver on each replica alone: same per-step work, summed walls
+    seq_wall, seq_work = 0.0, 0
+    for d in systems:
+        s = sm.Solver.from_system(d)
+        s.tune("graph", 1)
+        s.init_forces()
+        s.run(1, 16)
+        s.sync()
+        t0 = time.perf_counter()
+        s.run(17, seq_steps)
+        s.sync()
+        seq_wall += time.perf_counter() - t0
+        seq_work += len(d["type"]) * seq_steps
+        s.close()
+    seq = seq_work / seq_wall
+    log(f"sequential plain solver: {seq:.4g} particle-steps/s ({seq_steps} steps per replica)")
+    return {"value": batched, "unit": "particle-steps/s", "replicas": len(systems), "beads": nb,
+            "steps": steps, "warmup": warmup, "ms_per_step": 1e3 * wall / steps,
+            "sequential_value": seq, "sequential_steps_per_replica": seq_steps,
+            "speedup_vs_sequential": batched / seq, "autotuned": knobs,
+            "autotune_best_ms_per_step": tuned_ms,
+            "workload": "64 liposome replicas (seeds 1000..1063), lipid counts log-uniform in "
+                        "[2000, 40000] (MT19937 seed 20261202), tension linspace(0,1,8) x "
+                        "temperature 3*linspace(0.9,1.1,8), deltaLXY 0.01, MC box move every 8 steps"}
+
+
 def calib_copy():
     """Five copies of a 512 MiB buffer: the reference dispatch for scaling HBM counters."""
     import torch
@@ -141,6 +200,8 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=1000)
     ap.add_argument("--calib-copy", action="store_true")
     args = ap.parse_args()
     if args.calib_copy:
@@ -217,6 +278,9 @@ def main():
                      "algorithmic_bytes_per_launch": algo_bytes,
                      "valu_tflops": tflops, "valu_frac_fp32": tflops / VALU_FP32_PEAK_TFLOPS},
     }
+    s.close()
+    if not args.no_batched:
+        out["batched"] = bench_batched(args.batched_steps, 100, 200)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(n)

@@ -56,13 +56,24 @@ const char *sm_last_error(void);
 const char *sm_version(void);
 
 /* ---- context ---------------------------------------------------------------------- */
+/* A context holds R >= 1 replica systems stepped together, every kernel serving all of
+ * them in one launch.  Per-replica arrays below hold R entries (box: 3 per replica);
+ * bead arrays hold all beads, replica after replica (sm_replicas gives the ranges).
+ * A replica's results do not depend on the other replicas: its forces, energies and
+ * trajectory equal those of the same system in a context of its own. */
 /* Replaces state construction + toDevice (MDcuda.cu:218-278, include/cuda/state.h:11-97). */
 int sm_create(const sm_system *sys, sm_ctx **out);
+/* R replicas (the batched sweep; no reference counterpart -- the reference runs one MD
+ * process per system).  Replicas must share n_types, cutoff, dt and the constant tables;
+ * box, temperature, gamma, delta_lxy, tension and seed are per replica. */
+int sm_create_batch(const sm_system *systems, int32_t n_replicas, sm_ctx **out);
 int sm_destroy(sm_ctx *ctx);
+/* Replica count and each replica's bead range; any pointer may be NULL. */
+int sm_replicas(sm_ctx *ctx, int32_t *n_replicas, int32_t *first, int32_t *count);
 int sm_sync(sm_ctx *ctx);
-int sm_download_state(sm_ctx *ctx, double *pos, double *vel, double *acc, double box[3]);   /* state.toHost, state.h:77-97 */
-int sm_upload_state(sm_ctx *ctx, const double *pos, const double *vel, const double *acc, const double box[3]);
-int sm_set_temperature(sm_ctx *ctx, double T);   /* System.setInitialTemp (MD.cpp:369-371) */
+int sm_download_state(sm_ctx *ctx, double *pos, double *vel, double *acc, double *box);   /* state.toHost, state.h:77-97 */
+int sm_upload_state(sm_ctx *ctx, const double *pos, const double *vel, const double *acc, const double *box);
+int sm_set_temperature(sm_ctx *ctx, double T);   /* every replica; System.setInitialTemp (MD.cpp:369-371) */
 
 /* ---- the step ----------------------------------------------------------------------- */
 /* Step-0 forces (MD.cpp:186-262): a = F_pair + Langevin(T) + F_chain, no integration. */
@@ -73,26 +84,29 @@ int sm_second_half(sm_ctx *ctx);
  * reference happens right after this (MD.cpp:373-381). */
 int sm_step_first(sm_ctx *ctx, int64_t step);
 /* Rest of step `step` (MD.cpp:410-511, 589-721): Langevin, cell build, pair and CHAIN forces,
- * Verlet::second; if measure != NULL it receives {U_pair, U_chain, K} after the kick
- * (dataExtraction.h:827-1050); then the MC box move when step%8==0, step!=0, delta_lxy!=0. */
+ * Verlet::second; if measure != NULL it receives {U_pair, U_chain, K} of each replica after
+ * the kick (dataExtraction.h:827-1050); then the MC box move when step%8==0, step!=0, in
+ * every replica with delta_lxy != 0. */
 int sm_step_finish(sm_ctx *ctx, int64_t step, double *measure);
-/* Whole steps first_step .. first_step+nsteps-1 with no measurement, all on the device. */
+/* Whole steps first_step .. first_step+nsteps-1 with no measurement, all on the device.
+ * With the "graph" knob on, aligned blocks of 8 steps replay one captured hipGraph. */
 int sm_run(sm_ctx *ctx, int64_t first_step, int32_t nsteps);
-/* MC statistics so far (MD.cpp:713-720,748). */
+/* MC statistics so far, per replica (MD.cpp:713-720,748). */
 int sm_mc_stats(sm_ctx *ctx, int64_t *trials, int64_t *accepted);
-/* Accepted/rejected trial histograms over the fluctuation (MD.cpp:106-116,708-718): bin
- * width delta_lxy/101, bin b covers b*width - delta_lxy; n_out = int(2*101)+1 bins
- * (0 when delta_lxy == 0); at most `cap` bins are copied. */
-int sm_mc_hist(sm_ctx *ctx, int32_t cap, double *accepted, double *rejected, int32_t *n_out);
+/* Accepted/rejected trial histograms of one replica over the fluctuation
+ * (MD.cpp:106-116,708-718): bin width delta_lxy/101, bin b covers b*width - delta_lxy;
+ * n_out = int(2*101)+1 bins (0 when delta_lxy == 0); at most `cap` bins are copied. */
+int sm_mc_hist(sm_ctx *ctx, int32_t replica, int32_t cap, double *accepted, double *rejected, int32_t *n_out);
 
 /* ---- measurement and parity hooks --------------------------------------------------- */
-/* {U_pair, U_chain, K} in FP64 at the current state (dataExtraction.h:840-842,942-948). */
-int sm_measure(sm_ctx *ctx, double out[3]);
+/* {U_pair, U_chain, K} of each replica in FP64 at the current state
+ * (dataExtraction.h:840-842,942-948); out holds 3 per replica. */
+int sm_measure(sm_ctx *ctx, double *out);
 /* Conservative step-0 forces with FP32 pair arithmetic: pair part (CellOpt::computeForce,
  * cellOpt.h:712-926) and CHAIN part (Blob::doChainForce, system.h:1782-1866), n*3 each;
  * either pointer may be NULL. */
 int sm_compute_forces(sm_ctx *ctx, double *pair_out, double *chain_out);
-/* Cell key per particle (CellOpt::build, cellOpt.h:530-556). */
+/* Cell key per particle within its replica's grid (CellOpt::build, cellOpt.h:530-556). */
 int sm_cell_keys(sm_ctx *ctx, int32_t *keys);
 /* Neighbour count per particle with the r^2 < rc^2 test in FP64 (the FORCE_COUNT build
  * of Force, MD.h:838-845); total = number of pairs. */
@@ -100,14 +114,41 @@ int sm_pair_count(sm_ctx *ctx, int32_t *counts, int64_t *total);
 /* Pairs (i<j) within the cutoff, FP64 test, in no particular order. */
 int sm_pair_list(sm_ctx *ctx, int32_t *pairs, int64_t cap, int64_t *n_out);
 /* dU = U(old) - U(scaled) over the current pair set, FP64 (CellOpt::computeDPotential
- * cellOpt.h:1043-1180 and Blob::doChainDPotential system.h:3280-3384). */
-int sm_potential_delta(sm_ctx *ctx, const double scale[3], double *du_pair, double *du_chain);
+ * cellOpt.h:1043-1180 and Blob::doChainDPotential system.h:3280-3384); scale holds 3
+ * per replica, du_pair and du_chain one per replica. */
+int sm_potential_delta(sm_ctx *ctx, const double *scale, double *du_pair, double *du_chain);
 
 /* ---- launch shapes ------------------------------------------------------------------ */
 /* Set a launch-shape knob of this build; no counterpart in the reference (its launches are
  * fixed, include/cuda/other.h:6-11).  Knobs: "pair_chunk" = sorted slots per wave64 in
- * the pair-force kernel (1..4096). */
+ * the pair-force kernel (1..4096); "pair_variant" = 0 one-phase force kernel (default),
+ * 1 two-phase (FP32 screen into per-lane hit lists, then the FP64 test on the hits);
+ * "graph" = 1 to replay sm_run's aligned 8-step blocks as one captured hipGraph. */
 int sm_tune(sm_ctx *ctx, const char *knob, int32_t value);
+int sm_tune_get(sm_ctx *ctx, const char *knob, int32_t *value);
+/* Autotuner (no reference counterpart): times `steps` steps of this context's own work
+ * for each pair_chunk in {4,8,16,32,64} with the one-phase kernel and the graph on, then
+ * the best without the graph, from one saved state; keeps the fastest setting, restores
+ * the state, and returns the best ms per step.  These settings leave every result
+ * bit-identical, so tuning never changes a trajectory. */
+int sm_autotune(sm_ctx *ctx, int32_t steps, double *best_ms_per_step);
+
+/* ---- adjoint with respect to the interaction constants -------------------------------- */
+/* theta = {Umin, Umax} of the type pair (a, b) -- the laradjiRevaleeFC/PC rows (a,b) and
+ * (b,a), each keeping its r_m and r_c (potentials/laradjiRevalee.h:84-152) -- then k_bond
+ * and k_bend of every CHAIN constant set (molecule constants 1 and 3, system.h:1129-1158).
+ * No counterpart in the reference, whose constants are fixed per run; the callers are
+ * constant-fitting loops. */
+int sm_set_params(sm_ctx *ctx, int32_t type_a, int32_t type_b, const double *theta);
+int sm_get_params(sm_ctx *ctx, int32_t type_a, int32_t type_b, double *theta);
+/* From the context's current positions and velocities: initial forces, then steps
+ * 1..nsteps with the counter RNG (box moves as configured), J = mean over the steps of
+ * U_pair + U_chain after each step's kick.  If grad != NULL it receives dJ/dtheta (4
+ * values) from the discrete adjoint of the step in FP64, recomputing the trajectory from
+ * checkpoints every `checkpoint` steps; MC accept/reject decisions are held fixed (the
+ * pathwise derivative).  The context state is left as it was.  Single-system contexts. */
+int sm_adjoint(sm_ctx *ctx, int32_t type_a, int32_t type_b, int32_t nsteps, int32_t checkpoint, double *J,
+               double *grad);
 
 /* ---- profiling support -------------------------------------------------------------- */
 /* Average device time (ms) of each kernel class over the launches since the last reset,

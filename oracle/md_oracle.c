@@ -823,6 +823,27 @@ int32_t orc_md_run(orc_md *m, int32_t start, int32_t end)
 
 int32_t orc_md_trials(const orc_md *m) { return m->trials; }
 
+/* The objective of the constant-fitting adjoint: init forces, steps 1..nsteps of the loop
+ * above, J = mean over the steps of U_pair + U_chain taken after the kick and before the
+ * box move (where MD.cpp:513-548 measures). */
+double orc_md_objective(orc_md *m, int32_t nsteps)
+{
+    orc_sys *s = m->s;
+    orc_md_init_forces(m);
+    double J = 0.0;
+    for (int i = 1; i <= nsteps; i++) {
+        orc_verlet_first(s);
+        zero_acc(s);
+        thermostat(m, 0, (uint64_t)i);
+        orc_pair_force(s, s->acc);
+        orc_chain_force(s, s->acc);
+        orc_verlet_second(s);
+        J += orc_pair_potential(s) + orc_chain_potential(s);
+        if (i % 8 == 0 && s->delta_lxy != 0) mc_move(m, i);
+    }
+    return J / nsteps;
+}
+
 /* dataExtraction::compute potential and Kinetic (dataExtraction.h:840-842,942-948;
  * dataCollection.h:667-673). */
 void orc_md_measure(orc_md *m, double *U, double *K)

@@ -86,6 +86,7 @@ void orc_md_destroy(orc_md *m);
 void orc_md_init_forces(orc_md *m);            /* MD.cpp:186-262 */
 int32_t orc_md_run(orc_md *m, int32_t start, int32_t end); /* MD.cpp:335-729; returns #accepted MC moves */
 int32_t orc_md_trials(const orc_md *m);
+double orc_md_objective(orc_md *m, int32_t nsteps);         /* mean U after each kick, steps 1..nsteps */
 void orc_md_measure(orc_md *m, double *U, double *K);      /* dataExtraction.h:827-1050 */
 
 #ifdef __cplusplus

@@ -5,7 +5,8 @@ libsoftmold_hip.so).  The reference's operator interface for this path is the se
 `mpd::*_device(...)` templates driven by MDcuda.cu (include/cuda/mpdCuda.h:1-19) and the
 CellOpt/Verlet/Langevin/Blob classes driven by MD.cpp; `Solver` exposes the same
 operations (build + pair forces, CHAIN forces, Langevin, Verlet halves, potential, dU of a
-box trial, the MC move) on device-resident state.
+box trial, the MC move) on device-resident state, for one system or for a batch of
+replica systems stepped together (`Solver.batch`).
 
 There is no CPU fallback: if the HIP library is missing, importing the solver raises.
 """
@@ -47,13 +48,14 @@ class SmSystem(C.Structure):
 
 # every symbol the header declares (tests check the library exports all of them)
 EXPORTS = [
-    "sm_last_error", "sm_version", "sm_create", "sm_destroy", "sm_sync", "sm_download_state",
-    "sm_upload_state", "sm_set_temperature", "sm_init_forces", "sm_second_half",
-    "sm_step_first", "sm_step_finish", "sm_run", "sm_mc_stats", "sm_mc_hist", "sm_measure",
-    "sm_compute_forces", "sm_cell_keys", "sm_pair_count", "sm_pair_list",
-    "sm_potential_delta", "sm_tune", "sm_profile_enable", "sm_profile_read", "sm_mpd_read",
-    "sm_mpd_write", "sm_mpd_free", "sm_mpd_system", "sm_mpd_get", "sm_mpd_set",
-    "sm_mpd_set_state", "sm_gen_liposome", "sm_gen_bilayer",
+    "sm_last_error", "sm_version", "sm_create", "sm_create_batch", "sm_destroy", "sm_replicas",
+    "sm_sync", "sm_download_state", "sm_upload_state", "sm_set_temperature", "sm_init_forces",
+    "sm_second_half", "sm_step_first", "sm_step_finish", "sm_run", "sm_mc_stats", "sm_mc_hist",
+    "sm_measure", "sm_compute_forces", "sm_cell_keys", "sm_pair_count", "sm_pair_list",
+    "sm_potential_delta", "sm_tune", "sm_tune_get", "sm_autotune", "sm_set_params", "sm_get_params",
+    "sm_adjoint", "sm_profile_enable",
+    "sm_profile_read", "sm_mpd_read", "sm_mpd_write", "sm_mpd_free", "sm_mpd_system",
+    "sm_mpd_get", "sm_mpd_set", "sm_mpd_set_state", "sm_gen_liposome", "sm_gen_bilayer",
 ]
 
 _lib = None
@@ -79,7 +81,9 @@ def _declare(L):
         "sm_last_error": (C.c_char_p, []),
         "sm_version": (C.c_char_p, []),
         "sm_create": (C.c_int, [P(SmSystem), P(vp)]),
+        "sm_create_batch": (C.c_int, [P(SmSystem), i32, P(vp)]),
         "sm_destroy": (C.c_int, [vp]),
+        "sm_replicas": (C.c_int, [vp, P(i32), P(i32), P(i32)]),
         "sm_sync": (C.c_int, [vp]),
         "sm_download_state": (C.c_int, [vp, P(d), P(d), P(d), P(d)]),
         "sm_upload_state": (C.c_int, [vp, P(d), P(d), P(d), P(d)]),
@@ -90,7 +94,7 @@ def _declare(L):
         "sm_step_finish": (C.c_int, [vp, i64, P(d)]),
         "sm_run": (C.c_int, [vp, i64, i32]),
         "sm_mc_stats": (C.c_int, [vp, P(i64), P(i64)]),
-        "sm_mc_hist": (C.c_int, [vp, C.c_int32, P(d), P(d), P(i32)]),
+        "sm_mc_hist": (C.c_int, [vp, i32, i32, P(d), P(d), P(i32)]),
         "sm_measure": (C.c_int, [vp, P(d)]),
         "sm_compute_forces": (C.c_int, [vp, P(d), P(d)]),
         "sm_cell_keys": (C.c_int, [vp, P(i32)]),
@@ -98,6 +102,11 @@ def _declare(L):
         "sm_pair_list": (C.c_int, [vp, P(i32), i64, P(i64)]),
         "sm_potential_delta": (C.c_int, [vp, P(d), P(d), P(d)]),
         "sm_tune": (C.c_int, [vp, C.c_char_p, i32]),
+        "sm_tune_get": (C.c_int, [vp, C.c_char_p, P(i32)]),
+        "sm_autotune": (C.c_int, [vp, i32, P(d)]),
+        "sm_set_params": (C.c_int, [vp, i32, i32, P(d)]),
+        "sm_get_params": (C.c_int, [vp, i32, i32, P(d)]),
+        "sm_adjoint": (C.c_int, [vp, i32, i32, i32, i32, P(d), P(d)]),
         "sm_profile_enable": (C.c_int, [vp, i32]),
         "sm_profile_read": (C.c_int, [vp, i32, P(C.c_char_p), P(d), P(i64), P(i32)]),
         "sm_mpd_read": (C.c_int, [C.c_char_p, P(vp)]),
@@ -127,6 +136,10 @@ def _dp(a):
 
 def _ip(a):
     return a.ctypes.data_as(C.POINTER(C.c_int32)) if a is not None else None
+
+
+def _lp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64)) if a is not None else None
 
 
 class Mpd:
@@ -204,50 +217,90 @@ class Mpd:
             pass
 
 
+_SYS_KEYS = ("cutoff", "dt", "gamma", "temperature", "delta_lxy", "tension", "seed")
+
+
+def _pack(d: dict, keep: list) -> SmSystem:
+    """An SmSystem viewing numpy copies of a system dict (copies kept alive in `keep`)."""
+    k = {
+        "pos": np.ascontiguousarray(d["pos"], np.float64).reshape(-1, 3),
+        "type": np.ascontiguousarray(d["type"], np.int32),
+        "vel": np.ascontiguousarray(d["vel"], np.float64).reshape(-1, 3),
+        "fconst": np.ascontiguousarray(d["fconst"], np.float64),
+        "uconst": np.ascontiguousarray(d["uconst"], np.float64),
+        "chain": np.ascontiguousarray(d["chain"], np.int32).reshape(-1, 3),
+        "chain_const": np.ascontiguousarray(d["chain_const"], np.float64).reshape(-1, 4),
+    }
+    if d.get("acc") is not None:
+        k["acc"] = np.ascontiguousarray(d["acc"], np.float64).reshape(-1, 3)
+    keep.append(k)
+    s = SmSystem()
+    s.n_particles = len(k["type"])
+    s.n_types = int(d["n_types"])
+    s.pos, s.type, s.vel = _dp(k["pos"]), _ip(k["type"]), _dp(k["vel"])
+    s.acc = _dp(k["acc"]) if "acc" in k else None
+    s.n_chain_blocks = len(k["chain"])
+    s.chain, s.chain_const = _ip(k["chain"]), _dp(k["chain_const"])
+    s.fconst, s.uconst = _dp(k["fconst"]), _dp(k["uconst"])
+    s.box[0], s.box[1], s.box[2] = [float(x) for x in d["box"]]
+    s.cutoff, s.dt, s.gamma, s.temperature = d["cutoff"], d["dt"], d["gamma"], d["temperature"]
+    s.delta_lxy, s.tension = d["delta_lxy"], d["tension"]
+    s.seed = int(d["seed"]) & 0xFFFFFFFFFFFFFFFF
+    return s
+
+
 class Solver:
-    """Device-resident MD system driven through the C ABI."""
+    """Device-resident MD system(s) driven through the C ABI.
+
+    One system (`Solver(...)`, `from_system`, `from_mpd`) or R replica systems stepped
+    together (`Solver.batch([...])`).  Per-replica results come back with a leading
+    replica axis; for a single system that axis is dropped.
+    """
 
     def __init__(self, pos, type_, vel, box, fconst, uconst, n_types, chain, chain_const,
                  cutoff=2.0, dt=0.02, gamma=1.0, temperature=3.0, delta_lxy=0.0, tension=0.0,
                  seed=0, acc=None):
-        self._keep = {
-            "pos": np.ascontiguousarray(pos, np.float64).reshape(-1, 3),
-            "type": np.ascontiguousarray(type_, np.int32),
-            "vel": np.ascontiguousarray(vel, np.float64).reshape(-1, 3),
-            "fconst": np.ascontiguousarray(fconst, np.float64),
-            "uconst": np.ascontiguousarray(uconst, np.float64),
-            "chain": np.ascontiguousarray(chain, np.int32).reshape(-1, 3),
-            "chain_const": np.ascontiguousarray(chain_const, np.float64).reshape(-1, 4),
-        }
-        k = self._keep
-        if acc is not None:
-            k["acc"] = np.ascontiguousarray(acc, np.float64).reshape(-1, 3)
-        s = SmSystem()
-        s.n_particles = len(k["type"])
-        s.n_types = int(n_types)
-        s.pos, s.type, s.vel = _dp(k["pos"]), _ip(k["type"]), _dp(k["vel"])
-        s.acc = _dp(k["acc"]) if acc is not None else None
-        s.n_chain_blocks = len(k["chain"])
-        s.chain, s.chain_const = _ip(k["chain"]), _dp(k["chain_const"])
-        s.fconst, s.uconst = _dp(k["fconst"]), _dp(k["uconst"])
-        s.box[0], s.box[1], s.box[2] = [float(x) for x in box]
-        s.cutoff, s.dt, s.gamma, s.temperature = cutoff, dt, gamma, temperature
-        s.delta_lxy, s.tension, s.seed = delta_lxy, tension, int(seed) & 0xFFFFFFFFFFFFFFFF
-        self.n = s.n_particles
+        d = dict(pos=pos, type=type_, vel=vel, box=box, fconst=fconst, uconst=uconst,
+                 n_types=n_types, chain=chain, chain_const=chain_const, cutoff=cutoff, dt=dt,
+                 gamma=gamma, temperature=temperature, delta_lxy=delta_lxy, tension=tension,
+                 seed=seed, acc=acc)
+        self._create([d])
+
+    def _create(self, systems: list):
+        self._keep = []
+        arr = (SmSystem * len(systems))(*[_pack(d, self._keep) for d in systems])
         self.h = C.c_void_p()
-        _check(lib().sm_create(C.byref(s), C.byref(self.h)))
+        if len(systems) == 1:
+            _check(lib().sm_create(arr, C.byref(self.h)))
+        else:
+            _check(lib().sm_create_batch(arr, len(systems), C.byref(self.h)))
+        self.nrep = len(systems)
+        self.first = np.zeros(self.nrep, np.int32)
+        self.count = np.zeros(self.nrep, np.int32)
+        _check(lib().sm_replicas(self.h, None, _ip(self.first), _ip(self.count)))
+        self.n = int(self.count.sum())
 
     @classmethod
     def from_system(cls, d: dict, **over) -> "Solver":
-        kw = dict(cutoff=d["cutoff"], dt=d["dt"], gamma=d["gamma"], temperature=d["temperature"],
-                  delta_lxy=d["delta_lxy"], tension=d["tension"], seed=d["seed"])
-        kw.update(over)
-        return cls(d["pos"], d["type"], d["vel"], d["box"], d["fconst"], d["uconst"], d["n_types"],
-                   d["chain"], d["chain_const"], **kw)
+        d = dict(d)
+        d.update(over)
+        s = cls.__new__(cls)
+        s._create([d])
+        return s
 
     @classmethod
     def from_mpd(cls, m: Mpd, **over) -> "Solver":
         return cls.from_system(m.system(), **over)
+
+    @classmethod
+    def batch(cls, systems: list) -> "Solver":
+        """R replica systems sharing constants, cutoff and dt, stepped together."""
+        s = cls.__new__(cls)
+        s._create([dict(d) for d in systems])
+        return s
+
+    def _one(self, a):
+        return a[0] if self.nrep == 1 else a
 
     def close(self):
         if getattr(self, "h", None):
@@ -272,9 +325,9 @@ class Solver:
 
     def step_finish(self, step: int, measure: bool = False):
         if measure:
-            out = np.zeros(3)
+            out = np.zeros((self.nrep, 3))
             _check(lib().sm_step_finish(self.h, step, _dp(out)))
-            return out
+            return self._one(out)
         _check(lib().sm_step_finish(self.h, step, None))
         return None
 
@@ -288,22 +341,24 @@ class Solver:
         _check(lib().sm_set_temperature(self.h, T))
 
     def mc_stats(self):
-        t, a = C.c_int64(), C.c_int64()
-        _check(lib().sm_mc_stats(self.h, C.byref(t), C.byref(a)))
-        return t.value, a.value
+        t, a = np.zeros(self.nrep, np.int64), np.zeros(self.nrep, np.int64)
+        _check(lib().sm_mc_stats(self.h, _lp(t), _lp(a)))
+        if self.nrep == 1:
+            return int(t[0]), int(a[0])
+        return t, a
 
-    def mc_hist(self):
+    def mc_hist(self, replica: int = 0):
         """(accepted, rejected) trial histograms over the box fluctuation."""
         a, r, nb = np.zeros(256), np.zeros(256), C.c_int32()
-        _check(lib().sm_mc_hist(self.h, 256, _dp(a), _dp(r), C.byref(nb)))
+        _check(lib().sm_mc_hist(self.h, replica, 256, _dp(a), _dp(r), C.byref(nb)))
         return a[:nb.value].copy(), r[:nb.value].copy()
 
     # ---- state ----
     def download(self):
         n = self.n
-        pos, vel, acc, box = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(3)
+        pos, vel, acc, box = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((self.nrep, 3))
         _check(lib().sm_download_state(self.h, _dp(pos), _dp(vel), _dp(acc), _dp(box)))
-        return pos, vel, acc, box
+        return pos, vel, acc, self._one(box)
 
     def upload(self, pos=None, vel=None, acc=None, box=None):
         cv = lambda a: np.ascontiguousarray(a, np.float64) if a is not None else None
@@ -312,9 +367,10 @@ class Solver:
 
     # ---- measurement and parity hooks ----
     def measure(self):
-        out = np.zeros(3)
+        """(U_pair, U_chain, K) per replica, FP64."""
+        out = np.zeros((self.nrep, 3))
         _check(lib().sm_measure(self.h, _dp(out)))
-        return out  # U_pair, U_chain, K
+        return self._one(out)
 
     def compute_forces(self):
         n = self.n
@@ -343,13 +399,27 @@ class Solver:
         return buf[:nout.value]
 
     def potential_delta(self, scale):
-        s = np.ascontiguousarray(scale, np.float64)
-        a, b = C.c_double(), C.c_double()
-        _check(lib().sm_potential_delta(self.h, _dp(s), C.byref(a), C.byref(b)))
-        return a.value, b.value
+        """U(old) - U(scaled) (pair, chain); scale is (3,) or (R, 3)."""
+        s = np.ascontiguousarray(np.broadcast_to(np.asarray(scale, np.float64), (self.nrep, 3)))
+        a, b = np.zeros(self.nrep), np.zeros(self.nrep)
+        _check(lib().sm_potential_delta(self.h, _dp(s), _dp(a), _dp(b)))
+        if self.nrep == 1:
+            return float(a[0]), float(b[0])
+        return a, b
 
     def tune(self, knob: str, value: int):
         _check(lib().sm_tune(self.h, knob.encode(), int(value)))
+
+    def tune_get(self, knob: str) -> int:
+        v = C.c_int32()
+        _check(lib().sm_tune_get(self.h, knob.encode(), C.byref(v)))
+        return v.value
+
+    def autotune(self, steps: int = 64) -> float:
+        """Pick the fastest launch shape on this context's own work; ms per step."""
+        ms = C.c_double()
+        _check(lib().sm_autotune(self.h, steps, C.byref(ms)))
+        return ms.value
 
     # ---- profiling ----
     def profile(self, on: bool):
@@ -361,6 +431,5 @@ class Solver:
         avg = np.zeros(cap)
         cnt = np.zeros(cap, np.int64)
         nout = C.c_int32()
-        _check(lib().sm_profile_read(self.h, cap, names, _dp(avg),
-                                     cnt.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(nout)))
+        _check(lib().sm_profile_read(self.h, cap, names, _dp(avg), _lp(cnt), C.byref(nout)))
         return {names[i].decode(): (float(avg[i]), int(cnt[i])) for i in range(nout.value)}

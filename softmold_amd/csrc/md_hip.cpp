@@ -179,7 +179,7 @@ int run(const char *name)
         if (sys.delta_lxy != 0) {
             std::vector<double> ha(256), hr(256);
             int32_t nb = 0;
-            check(sm_mc_hist(ctx, 256, ha.data(), hr.data(), &nb), "mc histogram");
+            check(sm_mc_hist(ctx, 0, 256, ha.data(), hr.data(), &nb), "mc histogram");
             const double interval = sys.delta_lxy / 101.0;
             std::fstream f(fHist.c_str(), std::ios::out);
             for (int k = 0; k < nb; k++)

@@ -1,7 +1,9 @@
-// Host side of the MI355X MD-step solver: device context, step sequencing and the C ABI
-// declared in include/softmold_hip.h.
+// Host side of the MI355X MD-step solver: the device context (one or many replica
+// systems), step sequencing, hipGraph capture of the step, the launch-shape autotuner,
+// and the context part of the C ABI declared in include/softmold_hip.h.
 #include "../../include/softmold_hip.h"
-#include "mpd_io.hpp"
+#include "sm_adjoint.hpp"
+#include "sm_errors.hpp"
 #include "sm_internal.hpp"
 
 #include <hip/hip_runtime.h>
@@ -9,25 +11,17 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <functional>
-#include <stdexcept>
 #include <string>
 #include <vector>
 
 using namespace smdev;
+using namespace smapi;
 
-namespace smdev {
-void launch_reduce_sum(hipStream_t st, const double *partial, int count, double *out);
+namespace smapi {
+thread_local std::string g_err;
 }
 
 namespace {
-
-thread_local std::string g_err;
-
-struct ApiError : std::runtime_error {
-    int code;
-    ApiError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
-};
 
 #define HIPCHK(expr)                                                                                   \
     do {                                                                                               \
@@ -36,27 +30,6 @@ struct ApiError : std::runtime_error {
             throw ApiError(SM_ERR_HIP, std::string(hipGetErrorName(e_)) + " in " #expr " at " __FILE__ \
                                        ":" + std::to_string(__LINE__));                               \
     } while (0)
-
-template <class F>
-int guarded(F &&f)
-{
-    try {
-        f();
-        return SM_OK;
-    } catch (const ApiError &e) {
-        g_err = e.what();
-        return e.code;
-    } catch (const std::bad_alloc &) {
-        g_err = "host allocation failed";
-        return SM_ERR_INVALID;
-    } catch (const std::exception &e) {
-        g_err = e.what();
-        return SM_ERR_INVALID;
-    }
-}
-
-[[noreturn]] void invalid(const std::string &m) { throw ApiError(SM_ERR_INVALID, m); }
-[[noreturn]] void unsupported(const std::string &m) { throw ApiError(SM_ERR_UNSUPPORTED, m); }
 
 template <class T>
 T *dalloc(size_t count)
@@ -79,34 +52,74 @@ const char *kClassName[K_NCLASS] = {"integrate_first+key", "radix_sort", "reorde
                                     "bonded+langevin+kick", "pair_gather_fp64", "chain_energy_fp64",
                                     "kinetic_fp64", "mc_move"};
 
+// k_first's step argument: set the device step counter, advance it by one, or leave it
+constexpr long long kStepAdvance = -1, kStepKeep = -2;
+
+// host mirror of a replica's layout
+struct RepHost {
+    int first = 0, count = 0, cfirst = 0, ccount = 0;
+    long long capacity = 0, cellbase = 0;
+};
+
+// Tiles of a segment set: ranges [begin_r, begin_r + len_r) cut into kSegTile pieces that
+// start at each range's start.
+struct SegHost {
+    std::vector<int2> tile;
+    std::vector<int> rep_first;
+};
+
+SegHost make_tiles(const std::vector<int> &begin, const std::vector<int> &len)
+{
+    SegHost h;
+    for (size_t r = 0; r < begin.size(); r++) {
+        h.rep_first.push_back((int)h.tile.size());
+        for (int t = 0; t < len[r]; t += kSegTile)
+            h.tile.push_back(make_int2(begin[r] + t, begin[r] + std::min(len[r], t + kSegTile)));
+    }
+    h.rep_first.push_back((int)h.tile.size());
+    return h;
+}
+
 }  // namespace
 
 struct sm_ctx {
     hipStream_t st = nullptr;
     Params P{};
     int n = 0;
+    int nrep = 0;
     int nchains = 0;
-    double box0[3] = {0, 0, 0};
+    bool any_mc = false;   // some replica has delta_lxy != 0
+    std::vector<RepHost> rh;
     // device state
     Bead *beads = nullptr, *sbeads = nullptr;
     float4 *vel = nullptr, *acc = nullptr, *fpair = nullptr, *fscratch = nullptr;
     uint32_t *k0 = nullptr, *v0 = nullptr, *k1 = nullptr, *v1 = nullptr, *hist = nullptr;
     int *cstart = nullptr, *cend = nullptr;
-    long long capacity = 0;
+    long long capacity = 0;   // cells in the dense table, all replicas
     int sort_bits = 0;
-    DevBox *box = nullptr;
+    DevRep *reps = nullptr;
+    DevGlobal *glob = nullptr;
     float *fconstF = nullptr;
     double *uconstD = nullptr;
     int2 *chainPerBead = nullptr, *chainList = nullptr;
-    float4 *cconstF = nullptr;
     double4 *cconstD = nullptr;
-    double *partA = nullptr, *partB = nullptr, *partC = nullptr, *dres = nullptr;
+    // host copies of the constants (the adjoint changes them)
+    std::vector<double> hfconst, huconst;
+    std::vector<double4> hcconst;
+    double *adjF = nullptr, *adjDF = nullptr, *adjDU = nullptr;   // FP64 F table, d/dtheta tables
+    double *ebead = nullptr, *echain = nullptr, *dres = nullptr;   // dres: meas [R][3], then dU [R][2]
+    SegTiles segB{}, segC{};
     int *dcounts = nullptr, *dpairs = nullptr;
     long long pairs_cap = 0;
     unsigned long long *dnp = nullptr;
-    int chunk = 64;
+    int chunk = 8;
     int pair_variant = 0;
-    int sorted_sel = 0;  // which sort buffer pair holds the sorted result
+    int sorted_sel = 0;   // which sort buffer pair holds the sorted result
+    // captured 8-step graph
+    bool use_graph = false;
+    bool graph_valid = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
     // profiling
     bool prof = false;
     struct Rec { int cls; hipEvent_t a, b; };
@@ -117,17 +130,32 @@ struct sm_ctx {
 
     const uint32_t *skey() const { return sorted_sel ? k1 : k0; }
     const uint32_t *sidx() const { return sorted_sel ? v1 : v0; }
+    double *meas_dev() const { return dres; }
+    double *du_dev() const { return dres + 3 * (size_t)nrep; }
 
     ~sm_ctx()
     {
         if (st) (void)hipStreamSynchronize(st);
+        drop_graph();
         for (auto &r : recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
         for (auto e : pool) (void)hipEventDestroy(e);
         dfree(beads); dfree(sbeads); dfree(vel); dfree(acc); dfree(fpair); dfree(fscratch);
-        dfree(k0); dfree(v0); dfree(k1); dfree(v1); dfree(hist); dfree(cstart); dfree(cend); dfree(box);
-        dfree(fconstF); dfree(uconstD); dfree(chainPerBead); dfree(chainList); dfree(cconstF); dfree(cconstD);
-        dfree(partA); dfree(partB); dfree(partC); dfree(dres); dfree(dcounts); dfree(dpairs); dfree(dnp);
+        dfree(k0); dfree(v0); dfree(k1); dfree(v1); dfree(hist); dfree(cstart); dfree(cend);
+        dfree(reps); dfree(glob); dfree(fconstF); dfree(uconstD); dfree(chainPerBead); dfree(chainList);
+        dfree(cconstD); dfree(ebead); dfree(echain); dfree(dres); dfree(dcounts); dfree(dpairs); dfree(dnp);
+        dfree(adjF); dfree(adjDF); dfree(adjDU);
+        dfree(segB.tile); dfree(segB.rep_first); dfree(segB.tilesum);
+        dfree(segC.tile); dfree(segC.rep_first); dfree(segC.tilesum);
         if (st) (void)hipStreamDestroy(st);
+    }
+
+    void drop_graph()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+        graph_valid = false;
     }
 
     // ---- profiling helpers ----
@@ -170,133 +198,213 @@ struct sm_ctx {
         recs.clear();
     }
 
-    void check_launch()
-    {
-        HIPCHK(hipGetLastError());
-    }
+    void check_launch() { HIPCHK(hipGetLastError()); }
 
-    DevBox read_box()
+    std::vector<DevRep> read_reps()
     {
-        DevBox hb;
-        HIPCHK(hipMemcpyAsync(&hb, box, sizeof(DevBox), hipMemcpyDeviceToHost, st));
+        std::vector<DevRep> h(nrep);
+        HIPCHK(hipMemcpyAsync(h.data(), reps, sizeof(DevRep) * nrep, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        return hb;
+        return h;
+    }
+    void write_reps(const std::vector<DevRep> &h)
+    {
+        HIPCHK(hipMemcpyAsync(reps, h.data(), sizeof(DevRep) * nrep, hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    DevGlobal read_glob()
+    {
+        DevGlobal g;
+        HIPCHK(hipMemcpyAsync(&g, glob, sizeof g, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return g;
     }
 
     void check_flags()
     {
-        DevBox hb = read_box();
-        if (hb.flags & FLAG_BAD_POSITION)
+        const DevGlobal g = read_glob();
+        if (g.flags & FLAG_BAD_POSITION)
             throw ApiError(SM_ERR_STATE, "a particle position left the box or became non-finite");
-        if (hb.flags & FLAG_SORT_TIMEOUT)
+        if (g.flags & FLAG_SORT_TIMEOUT)
             throw ApiError(SM_ERR_STATE, "cell-key sort look-back timed out");
-        if (hb.flags & FLAG_CELL_OVERFLOW)
+        if (g.flags & FLAG_CELL_OVERFLOW)
             throw ApiError(SM_ERR_STATE, "an accepted box move would exceed the cell table or make fewer than 3 cells per axis");
     }
 
-    // Dense cell table sized for every box the next `moves` MC moves can reach.
+    // Dense cell table sized for every box the next `moves` MC moves can reach, in every
+    // replica; a replica that outgrows its reservation moves every cell base (keys are
+    // recomputed each step, so only the table is reallocated).
     void ensure_capacity(long long moves)
     {
-        DevBox hb = read_box();
-        const double rc = P.cutoff, dL = P.delta_lxy;
-        const double grow = (double)moves * std::fabs(dL);
-        double Lmax[3], Lmin[2];
-        for (int d = 0; d < 2; d++) {
-            Lmax[d] = hb.L[d] + grow;
-            Lmin[d] = std::max(hb.L[d] - grow, 1e-300);
+        std::vector<DevRep> hr = read_reps();
+        bool relayout = capacity == 0;
+        const double rc = P.cutoff;
+        for (int r = 0; r < nrep; r++) {
+            const double dL = hr[r].delta_lxy;
+            const double grow = (double)moves * std::fabs(dL);
+            double Lmax[3], Lmin[2];
+            for (int d = 0; d < 2; d++) {
+                Lmax[d] = hr[r].L[d] + grow;
+                Lmin[d] = std::max(hr[r].L[d] - grow, 1e-300);
+            }
+            Lmax[2] = (moves > 0 && dL != 0) ? hr[r].L[2] * (hr[r].L[0] * hr[r].L[1]) / (Lmin[0] * Lmin[1]) : hr[r].L[2];
+            long long need = 1;
+            for (int d = 0; d < 3; d++) need *= (long long)(Lmax[d] / rc) + 1;
+            if (need > rh[r].capacity) {
+                rh[r].capacity = need + need / 8;
+                relayout = true;
+            }
         }
-        Lmax[2] = (moves > 0 && dL != 0) ? hb.L[2] * (hb.L[0] * hb.L[1]) / (Lmin[0] * Lmin[1]) : hb.L[2];
-        long long need = 1;
-        for (int d = 0; d < 3; d++) need *= (long long)(Lmax[d] / rc) + 1;
-        if (need <= capacity) return;
-        long long cap = need + need / 8;
+        if (!relayout) return;
+        long long base = 0;
+        for (int r = 0; r < nrep; r++) {
+            rh[r].cellbase = base;
+            base += rh[r].capacity;
+            hr[r].cellbase = rh[r].cellbase;
+            hr[r].capacity = rh[r].capacity;
+        }
+        if (base > 0xFFFFFFFFll) unsupported("cell tables of all replicas exceed 2^32 cells");
+        drop_graph();
         dfree(cstart);
         dfree(cend);
-        cstart = dalloc<int>((size_t)cap);
-        cend = dalloc<int>((size_t)cap);
-        HIPCHK(hipMemsetAsync(cstart, 0xFF, sizeof(int) * (size_t)cap, st));
-        HIPCHK(hipMemsetAsync(cend, 0xFF, sizeof(int) * (size_t)cap, st));
-        capacity = cap;
-        HIPCHK(hipMemcpyAsync(&box->capacity, &capacity, sizeof(long long), hipMemcpyHostToDevice, st));
+        cstart = dalloc<int>((size_t)base);
+        cend = dalloc<int>((size_t)base);
+        HIPCHK(hipMemsetAsync(cstart, 0xFF, sizeof(int) * (size_t)base, st));
+        HIPCHK(hipMemsetAsync(cend, 0xFF, sizeof(int) * (size_t)base, st));
+        capacity = base;
+        write_reps(hr);
         int bits = 1;
-        while ((1ll << bits) < cap) bits++;
+        while ((1ll << bits) < base) bits++;
         sort_bits = bits;
     }
 
     // keys (optionally after Verlet::first), sort, reorder + dense cell table
-    void first_and_keys(int integrate)
+    void first_and_keys(int integrate, long long step_set)
     {
-        timed(K_FIRST, [&] { launch_first(st, P, beads, vel, acc, box, k0, v0, integrate); });
+        timed(K_FIRST, [&] { launch_first(st, P, beads, vel, acc, reps, glob, k0, v0, integrate, step_set); });
     }
     void sort_and_reorder()
     {
         timed(K_SORT, [&] {
-            SortBuffers sb{k0, v0, k1, v1, hist, &box->flags, n, 0};
+            SortBuffers sb{k0, v0, k1, v1, hist, &glob->flags, n, 0};
             sorted_sel = radix_sort(st, sb, sort_bits);
         });
         timed(K_REORDER, [&] { launch_reorder(st, n, skey(), sidx(), beads, sbeads, cstart, cend); });
     }
-    void build(int integrate)
+    void build(int integrate, long long step_set)
     {
-        first_and_keys(integrate);
+        first_and_keys(integrate, step_set);
         sort_and_reorder();
     }
     void pair_force()
     {
         timed(K_PAIR, [&] {
-            launch_pair_force(st, P, chunk, pair_variant, sbeads, skey(), sidx(), cstart, cend, box, fconstF, fpair);
+            launch_pair_force(st, P, chunk, pair_variant, sbeads, skey(), sidx(), cstart, cend, reps, fconstF, fpair);
         });
     }
-    void bonded(long long step, int mode, float4 *fchain_out)
+    void bonded(int mode, float4 *fchain_out)
     {
         timed(K_BONDED, [&] {
-            launch_bonded_kick(st, P, beads, chainPerBead, cconstD, fpair, vel, acc, box, step, mode, fchain_out);
+            launch_bonded_kick(st, P, beads, chainPerBead, cconstD, fpair, vel, acc, reps, glob, mode, fchain_out);
         });
     }
-    // {U_pair, U_chain, K} into dres[0..2] (device)
+    // {U_pair, U_chain, K} of every replica into dres[3r + 0..2] (device)
     void measure_device()
     {
         timed(K_GATHER, [&] {
-            launch_pair_gather(st, P, chunk, GATHER_U, sbeads, skey(), sidx(), cstart, cend, box, uconstD, partA, nullptr,
-                               nullptr, nullptr, 0);
+            launch_pair_gather(st, P, chunk, GATHER_U, sbeads, skey(), sidx(), cstart, cend, reps, uconstD, ebead,
+                               nullptr, nullptr, nullptr, 0);
+            launch_seg_sum(st, segB, ebead, meas_dev() + 0, 3);
         });
-        timed(K_CHAIN_E, [&] { launch_chain_energy(st, nchains, chainList, cconstD, beads, box, 0, partB); });
-        timed(K_KIN, [&] { launch_kinetic(st, n, vel, partC); });
-        launch_reduce_sum(st, partA, cells_blocks(n, chunk), dres + 0);
-        launch_reduce_sum(st, partB, chain_blocks(nchains), dres + 1);
-        launch_reduce_sum(st, partC, gather_blocks(n), dres + 2);
+        timed(K_CHAIN_E, [&] {
+            launch_chain_energy(st, nchains, chainList, cconstD, beads, reps, 0, echain);
+            launch_seg_sum(st, segC, echain, meas_dev() + 1, 3);
+        });
+        timed(K_KIN, [&] {
+            launch_kinetic(st, n, vel, ebead);
+            launch_seg_sum(st, segB, ebead, meas_dev() + 2, 3);
+        });
     }
+    // U(old) - U(scaled) of every replica with its trial scale into dres[3R + 2r + 0..1]
     void delta_device()
     {
         timed(K_GATHER, [&] {
-            launch_pair_gather(st, P, chunk, GATHER_DU, sbeads, skey(), sidx(), cstart, cend, box, uconstD, partA, nullptr,
-                               nullptr, nullptr, 0);
+            launch_pair_gather(st, P, chunk, GATHER_DU, sbeads, skey(), sidx(), cstart, cend, reps, uconstD, ebead,
+                               nullptr, nullptr, nullptr, 0);
+            launch_seg_sum(st, segB, ebead, du_dev() + 0, 2);
         });
-        timed(K_CHAIN_E, [&] { launch_chain_energy(st, nchains, chainList, cconstD, beads, box, 1, partB); });
+        timed(K_CHAIN_E, [&] {
+            launch_chain_energy(st, nchains, chainList, cconstD, beads, reps, 1, echain);
+            launch_seg_sum(st, segC, echain, du_dev() + 1, 2);
+        });
     }
-    void mc_move(long long step)
+    void mc_move()
     {
-        timed(K_MC, [&] { launch_mc_propose(st, P, box, step); });
+        timed(K_MC, [&] { launch_mc_propose(st, nrep, reps, glob); });
         delta_device();
         timed(K_MC, [&] {
-            launch_mc_decide(st, P, box, partA, cells_blocks(n, chunk), partB, chain_blocks(nchains));
-            launch_rescale(st, n, beads, box);
+            launch_mc_decide(st, P, reps, glob, du_dev());
+            launch_rescale(st, n, beads, reps);
         });
     }
-    bool mc_step(long long step) const { return step % 8 == 0 && step != 0 && P.delta_lxy != 0; }
+    bool mc_step(long long step) const { return step % 8 == 0 && step != 0 && any_mc; }
 
-    void step_first(long long step)
-    {
-        (void)step;
-        first_and_keys(1);
-    }
-    void step_finish(long long step, bool measure)
+    void step_first(long long step_set) { first_and_keys(1, step_set); }
+    void step_finish(bool measure, bool mc)
     {
         sort_and_reorder();
         pair_force();
-        bonded(step, KICK_STEP, nullptr);
+        bonded(KICK_STEP, nullptr);
         if (measure) measure_device();
-        if (mc_step(step)) mc_move(step);
+        if (mc) mc_move();
+    }
+    void eager_step(long long step)
+    {
+        step_first(step);
+        step_finish(false, mc_step(step));
+    }
+
+    // Steps s .. s+7 with s % 8 == 1 as one graph: the step counter advances on the
+    // device, so the same graph serves every such block; the MC move (if any replica
+    // has one) falls on the block's last step.
+    void graph_block(long long s)
+    {
+        if (!graph_valid) {
+            drop_graph();
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < 8; k++) {
+                step_first(kStepAdvance);
+                step_finish(false, k == 7 && any_mc);
+            }
+            hipError_t e = hipStreamEndCapture(st, &graph);
+            if (e != hipSuccess) {
+                graph = nullptr;
+                HIPCHK(e);
+            }
+            HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+            graph_valid = true;
+        }
+        // the block advances from step s - 1
+        launch_set_step(st, glob, s - 1);
+        HIPCHK(hipGraphLaunch(gexec, st));
+    }
+
+    void run(long long first, long long nsteps)
+    {
+        long long moves = 0;
+        for (long long s = first; s < first + nsteps; s++)
+            if (mc_step(s)) moves++;
+        if (moves) ensure_capacity(moves);
+        long long s = first;
+        const long long end = first + nsteps;
+        if (use_graph && !prof) {
+            while (s < end && s % 8 != 1) eager_step(s++);
+            while (end - s >= 8) {
+                graph_block(s);
+                s += 8;
+            }
+        }
+        while (s < end) eager_step(s++);
     }
 };
 
@@ -342,83 +450,129 @@ void validate(const sm_system *s)
     if (s->n_chain_blocks > 0 && (!s->chain || !s->chain_const)) invalid("chain blocks without data");
 }
 
-}  // namespace
-
-extern "C" {
-
-const char *sm_last_error(void) { return g_err.c_str(); }
-const char *sm_version(void) { return "softmold-mi355x 0.1 (gfx950)"; }
-
-int sm_create(const sm_system *s, sm_ctx **out)
+// replicas of one context share the interaction constants, cutoff and time step
+void validate_batch(const sm_system *s, int nrep)
 {
-    if (!out) {
-        g_err = "null output pointer";
-        return SM_ERR_INVALID;
+    long long total = 0;
+    for (int r = 0; r < nrep; r++) {
+        validate(&s[r]);
+        total += s[r].n_particles;
+        if (r == 0) continue;
+        const sm_system &a = s[0], &b = s[r];
+        if (b.n_types != a.n_types || b.cutoff != a.cutoff || b.dt != a.dt)
+            unsupported("replicas must share n_types, cutoff and dt");
+        const size_t nt = 6 * (size_t)a.n_types * a.n_types;
+        if (std::memcmp(a.fconst, b.fconst, nt * sizeof(double)) != 0 ||
+            std::memcmp(a.uconst, b.uconst, nt * sizeof(double)) != 0)
+            unsupported("replicas must share the two-body constant tables");
     }
-    *out = nullptr;
-    sm_ctx *c = nullptr;
-    int rc = guarded([&] {
-        validate(s);
-        c = new sm_ctx();
-        const int n = s->n_particles;
+    if (total > 0x7FFFFFFF) unsupported("more than 2^31-1 beads in one context");
+}
+
+sm_ctx *create(const sm_system *s, int nrep)
+{
+    validate_batch(s, nrep);
+    sm_ctx *c = new sm_ctx();
+    try {
+        c->nrep = nrep;
+        c->rh.resize(nrep);
+        int n = 0, nch = 0;
+        for (int r = 0; r < nrep; r++) {
+            c->rh[r].first = n;
+            c->rh[r].count = s[r].n_particles;
+            n += s[r].n_particles;
+        }
         c->n = n;
         c->P.n = n;
-        c->P.nT = s->n_types;
-        c->P.cutoff = s->cutoff;
-        c->P.dt = s->dt;
-        c->P.gamma = s->gamma;
-        c->P.temperature = s->temperature;
-        c->P.delta_lxy = s->delta_lxy;
-        c->P.tension = s->tension;
-        c->P.seed = s->seed;
-        for (int d = 0; d < 3; d++) c->box0[d] = s->box[d];
+        c->P.nrep = nrep;
+        c->P.nT = s[0].n_types;
+        c->P.cutoff = s[0].cutoff;
+        c->P.dt = s[0].dt;
         HIPCHK(hipStreamCreate(&c->st));
 
-        // chains: per-bead role table and per-chain list
+        // chains: per-bead role table and per-chain list, replica after replica
         std::vector<int2> perBead(n, make_int2(-1, 0));
         std::vector<int2> list;
-        std::vector<float4> ccF;
         std::vector<double4> ccD;
-        for (int b = 0; b < s->n_chain_blocks; b++) {
-            const int start = s->chain[3 * b], nch = s->chain[3 * b + 1], len = s->chain[3 * b + 2];
-            if (len < 3 || len > 0xFFFF) unsupported("CHAIN length must be in [3, 65535]");
-            if (nch < 0 || start < 0 || (long long)start + (long long)nch * len > n) invalid("CHAIN block out of range");
-            const double *cc = s->chain_const + 4 * b;
-            const int cid = (int)ccF.size();
-            if (cid > 0x7FFF) unsupported("too many CHAIN constant sets");
-            ccF.push_back(make_float4((float)cc[0], (float)cc[1], (float)cc[2], (float)cc[3]));
-            ccD.push_back(make_double4(cc[0], cc[1], cc[2], cc[3]));
-            for (int k = 0; k < nch; k++) {
-                const int st0 = start + k * len;
-                list.push_back(make_int2(st0, len | (cid << 16)));
-                for (int j = 0; j < len; j++) {
-                    if (perBead[st0 + j].x != -1) unsupported("a bead belongs to two CHAIN records");
-                    perBead[st0 + j] = make_int2(st0, len | (cid << 16));
+        for (int r = 0; r < nrep; r++) {
+            const int off = c->rh[r].first, nr = s[r].n_particles;
+            c->rh[r].cfirst = (int)list.size();
+            for (int b = 0; b < s[r].n_chain_blocks; b++) {
+                const int start = s[r].chain[3 * b], cnt = s[r].chain[3 * b + 1], len = s[r].chain[3 * b + 2];
+                if (len < 3 || len > 0xFFFF) unsupported("CHAIN length must be in [3, 65535]");
+                if (cnt < 0 || start < 0 || (long long)start + (long long)cnt * len > nr)
+                    invalid("CHAIN block out of range");
+                const double *cc = s[r].chain_const + 4 * b;
+                const int cid = (int)ccD.size();
+                if (cid > 0x7FFF) unsupported("too many CHAIN constant sets");
+                ccD.push_back(make_double4(cc[0], cc[1], cc[2], cc[3]));
+                for (int k = 0; k < cnt; k++) {
+                    const int st0 = off + start + k * len;
+                    list.push_back(make_int2(st0, len | (cid << 16)));
+                    for (int j = 0; j < len; j++) {
+                        if (perBead[st0 + j].x != -1) unsupported("a bead belongs to two CHAIN records");
+                        perBead[st0 + j] = make_int2(st0, len | (cid << 16));
+                    }
                 }
             }
+            c->rh[r].ccount = (int)list.size() - c->rh[r].cfirst;
         }
-        c->nchains = (int)list.size();
+        nch = (int)list.size();
+        c->nchains = nch;
 
         // host staging
         std::vector<Bead> hb(n);
         std::vector<float4> hv(n), ha(n);
-        for (int i = 0; i < n; i++) {
-            hb[i].x = s->pos[3 * (size_t)i];
-            hb[i].y = s->pos[3 * (size_t)i + 1];
-            hb[i].z = s->pos[3 * (size_t)i + 2];
-            hb[i].type = s->type[i];
-            hb[i].pad = 0;
-            hv[i] = make_float4((float)s->vel[3 * (size_t)i], (float)s->vel[3 * (size_t)i + 1],
-                                (float)s->vel[3 * (size_t)i + 2], 0.f);
-            if (s->acc)
-                ha[i] = make_float4((float)s->acc[3 * (size_t)i], (float)s->acc[3 * (size_t)i + 1],
-                                    (float)s->acc[3 * (size_t)i + 2], 0.f);
-            else
-                ha[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        std::vector<DevRep> hr(nrep);
+        for (int r = 0; r < nrep; r++) {
+            const sm_system &q = s[r];
+            const int off = c->rh[r].first;
+            for (int i = 0; i < q.n_particles; i++) {
+                const size_t k = 3 * (size_t)i;
+                Bead &b = hb[off + i];
+                b.x = q.pos[k];
+                b.y = q.pos[k + 1];
+                b.z = q.pos[k + 2];
+                b.type = q.type[i];
+                b.rep = r;
+                hv[off + i] = make_float4((float)q.vel[k], (float)q.vel[k + 1], (float)q.vel[k + 2], 0.f);
+                ha[off + i] = q.acc ? make_float4((float)q.acc[k], (float)q.acc[k + 1], (float)q.acc[k + 2], 0.f)
+                                    : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            DevRep &R = hr[r];
+            std::memset(&R, 0, sizeof R);
+            for (int d = 0; d < 3; d++) {
+                R.L[d] = q.box[d];
+                R.nc[d] = (int)(q.box[d] / q.cutoff);
+                R.cs[d] = q.box[d] / R.nc[d];
+                R.scale[d] = 1.0;
+            }
+            R.first = c->rh[r].first;
+            R.count = c->rh[r].count;
+            R.cfirst = c->rh[r].cfirst;
+            R.ccount = c->rh[r].ccount;
+            R.seed = q.seed;
+            R.temperature = q.temperature;
+            R.gamma = q.gamma;
+            R.sigma = std::sqrt((6.0 * q.temperature * q.gamma) / q.dt);
+            R.delta_lxy = q.delta_lxy;
+            R.tension = q.tension;
+            if (q.delta_lxy != 0) c->any_mc = true;
         }
-        const int nT = s->n_types;
+        const int nT = s[0].n_types;
         std::vector<float> fF(6 * nT * nT);
-        for (size_t k = 0; k < fF.size(); k++) fF[k] = (float)s->fconst[k];
+        for (size_t k = 0; k < fF.size(); k++) fF[k] = (float)s[0].fconst[k];
+        c->hfconst.assign(s[0].fconst, s[0].fconst + fF.size());
+        c->huconst.assign(s[0].uconst, s[0].uconst + fF.size());
+        c->hcconst = ccD;
+
+        // segment tiles: beads (molecule order == sorted order per replica) and chains
+        std::vector<int> bb(nrep), bl(nrep), cb(nrep), cl(nrep);
+        for (int r = 0; r < nrep; r++) {
+            bb[r] = c->rh[r].first; bl[r] = c->rh[r].count;
+            cb[r] = c->rh[r].cfirst; cl[r] = c->rh[r].ccount;
+        }
+        const SegHost tb = make_tiles(bb, bl), tc = make_tiles(cb, cl);
 
         c->beads = dalloc<Bead>(n);
         c->sbeads = dalloc<Bead>(n);
@@ -431,59 +585,119 @@ int sm_create(const sm_system *s, sm_ctx **out)
         c->k1 = dalloc<uint32_t>(n);
         c->v1 = dalloc<uint32_t>(n);
         c->hist = dalloc<uint32_t>(sort_scratch_words(n, 4));
-        c->box = dalloc<DevBox>(1);
+        c->reps = dalloc<DevRep>(nrep);
+        c->glob = dalloc<DevGlobal>(1);
         c->fconstF = dalloc<float>(fF.size());
         c->uconstD = dalloc<double>(fF.size());
         c->chainPerBead = dalloc<int2>(n);
         c->chainList = dalloc<int2>(list.size());
-        c->cconstF = dalloc<float4>(ccF.size());
         c->cconstD = dalloc<double4>(ccD.size());
-        const int gb = gather_blocks(n);
-        c->partA = dalloc<double>(cells_blocks(n, 1));
-        c->partB = dalloc<double>(chain_blocks(c->nchains));
-        c->partC = dalloc<double>(gb);
-        c->dres = dalloc<double>(4);
+        c->ebead = dalloc<double>(n);
+        c->echain = dalloc<double>(nch);
+        c->dres = dalloc<double>(5 * (size_t)nrep);
         c->dcounts = dalloc<int>(n);
         c->dnp = dalloc<unsigned long long>(1);
+        auto upseg = [&](SegTiles &S, const SegHost &h) {
+            S.ntiles = (int)h.tile.size();
+            S.nrep = nrep;
+            S.tile = dalloc<int2>(h.tile.size());
+            S.rep_first = dalloc<int>(h.rep_first.size());
+            S.tilesum = dalloc<double>(h.tile.size());
+            if (!h.tile.empty())
+                HIPCHK(hipMemcpyAsync(S.tile, h.tile.data(), sizeof(int2) * h.tile.size(), hipMemcpyHostToDevice, c->st));
+            HIPCHK(hipMemcpyAsync(S.rep_first, h.rep_first.data(), sizeof(int) * h.rep_first.size(),
+                                  hipMemcpyHostToDevice, c->st));
+        };
+        upseg(c->segB, tb);
+        upseg(c->segC, tc);
 
-        DevBox db;
-        std::memset(&db, 0, sizeof db);
-        for (int d = 0; d < 3; d++) {
-            db.L[d] = s->box[d];
-            db.nc[d] = (int)(s->box[d] / s->cutoff);
-            db.cs[d] = s->box[d] / db.nc[d];
-            db.scale[d] = 1.0;
-        }
+        DevGlobal g{};
         hipStream_t st = c->st;
         HIPCHK(hipMemcpyAsync(c->beads, hb.data(), sizeof(Bead) * n, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(c->vel, hv.data(), sizeof(float4) * n, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(c->acc, ha.data(), sizeof(float4) * n, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemsetAsync(c->fpair, 0, sizeof(float4) * n, st));
-        HIPCHK(hipMemcpyAsync(c->box, &db, sizeof(DevBox), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(c->reps, hr.data(), sizeof(DevRep) * nrep, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(c->glob, &g, sizeof g, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(c->fconstF, fF.data(), sizeof(float) * fF.size(), hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(c->uconstD, s->uconst, sizeof(double) * fF.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(c->uconstD, s[0].uconst, sizeof(double) * fF.size(), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(c->chainPerBead, perBead.data(), sizeof(int2) * n, hipMemcpyHostToDevice, st));
         if (!list.empty()) {
             HIPCHK(hipMemcpyAsync(c->chainList, list.data(), sizeof(int2) * list.size(), hipMemcpyHostToDevice, st));
-            HIPCHK(hipMemcpyAsync(c->cconstF, ccF.data(), sizeof(float4) * ccF.size(), hipMemcpyHostToDevice, st));
             HIPCHK(hipMemcpyAsync(c->cconstD, ccD.data(), sizeof(double4) * ccD.size(), hipMemcpyHostToDevice, st));
         }
         // small systems: shorter chunks so the pair kernel still fills the chip
         c->chunk = (n >= 100000) ? 16 : 8;
         c->ensure_capacity(0);
         HIPCHK(hipStreamSynchronize(st));
-    });
-    if (rc != SM_OK) {
+    } catch (...) {
         delete c;
-        return rc;
+        throw;
     }
-    *out = c;
-    return SM_OK;
+    return c;
+}
+
+template <class T>
+void download_f4(sm_ctx *c, const float4 *src, T *dst)
+{
+    std::vector<float4> h(c->n);
+    HIPCHK(hipMemcpyAsync(h.data(), src, sizeof(float4) * c->n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (int i = 0; i < c->n; i++) {
+        dst[3 * (size_t)i] = h[i].x;
+        dst[3 * (size_t)i + 1] = h[i].y;
+        dst[3 * (size_t)i + 2] = h[i].z;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------
+extern "C" {
+
+const char *sm_last_error(void) { return g_err.c_str(); }
+const char *sm_version(void) { return "softmold-mi355x 0.2 (gfx950)"; }
+
+int sm_create(const sm_system *s, sm_ctx **out)
+{
+    if (!out) {
+        g_err = "null output pointer";
+        return SM_ERR_INVALID;
+    }
+    *out = nullptr;
+    return guarded([&] { *out = create(s, 1); });
+}
+
+int sm_create_batch(const sm_system *systems, int32_t n_replicas, sm_ctx **out)
+{
+    if (!out) {
+        g_err = "null output pointer";
+        return SM_ERR_INVALID;
+    }
+    *out = nullptr;
+    return guarded([&] {
+        if (!systems || n_replicas < 1) invalid("need at least one replica");
+        *out = create(systems, n_replicas);
+    });
 }
 
 int sm_destroy(sm_ctx *c)
 {
     return guarded([&] { delete c; });
+}
+
+int sm_replicas(sm_ctx *c, int32_t *n_replicas, int32_t *first, int32_t *count)
+{
+    return guarded([&] {
+        if (!c) invalid("null context");
+        if (n_replicas) *n_replicas = c->nrep;
+        for (int r = 0; r < c->nrep; r++) {
+            if (first) first[r] = c->rh[r].first;
+            if (count) count[r] = c->rh[r].count;
+        }
+    });
 }
 
 int sm_sync(sm_ctx *c)
@@ -495,50 +709,58 @@ int sm_sync(sm_ctx *c)
     });
 }
 
-int sm_download_state(sm_ctx *c, double *pos, double *vel, double *acc, double box[3])
+int sm_download_state(sm_ctx *c, double *pos, double *vel, double *acc, double *box)
 {
     return guarded([&] {
         if (!c) invalid("null context");
         const int n = c->n;
-        std::vector<Bead> hb(pos ? n : 0);
-        std::vector<float4> hv(vel ? n : 0), ha(acc ? n : 0);
-        if (pos) HIPCHK(hipMemcpyAsync(hb.data(), c->beads, sizeof(Bead) * n, hipMemcpyDeviceToHost, c->st));
-        if (vel) HIPCHK(hipMemcpyAsync(hv.data(), c->vel, sizeof(float4) * n, hipMemcpyDeviceToHost, c->st));
-        if (acc) HIPCHK(hipMemcpyAsync(ha.data(), c->acc, sizeof(float4) * n, hipMemcpyDeviceToHost, c->st));
-        DevBox b = c->read_box();
-        for (int i = 0; i < n; i++) {
-            if (pos) { pos[3 * (size_t)i] = hb[i].x; pos[3 * (size_t)i + 1] = hb[i].y; pos[3 * (size_t)i + 2] = hb[i].z; }
-            if (vel) { vel[3 * (size_t)i] = hv[i].x; vel[3 * (size_t)i + 1] = hv[i].y; vel[3 * (size_t)i + 2] = hv[i].z; }
-            if (acc) { acc[3 * (size_t)i] = ha[i].x; acc[3 * (size_t)i + 1] = ha[i].y; acc[3 * (size_t)i + 2] = ha[i].z; }
+        if (pos) {
+            std::vector<Bead> hb(n);
+            HIPCHK(hipMemcpyAsync(hb.data(), c->beads, sizeof(Bead) * n, hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+            for (int i = 0; i < n; i++) {
+                pos[3 * (size_t)i] = hb[i].x;
+                pos[3 * (size_t)i + 1] = hb[i].y;
+                pos[3 * (size_t)i + 2] = hb[i].z;
+            }
         }
-        if (box) for (int d = 0; d < 3; d++) box[d] = b.L[d];
-        if (b.flags) c->check_flags();
+        if (vel) download_f4(c, c->vel, vel);
+        if (acc) download_f4(c, c->acc, acc);
+        std::vector<DevRep> hr = c->read_reps();
+        if (box)
+            for (int r = 0; r < c->nrep; r++)
+                for (int d = 0; d < 3; d++) box[3 * r + d] = hr[r].L[d];
+        c->check_flags();
     });
 }
 
-int sm_upload_state(sm_ctx *c, const double *pos, const double *vel, const double *acc, const double box[3])
+int sm_upload_state(sm_ctx *c, const double *pos, const double *vel, const double *acc, const double *box)
 {
     return guarded([&] {
         if (!c) invalid("null context");
         const int n = c->n;
-        DevBox b = c->read_box();
+        std::vector<DevRep> hr = c->read_reps();
         if (box) {
-            for (int d = 0; d < 3; d++) {
-                if (!(box[d] > 0)) invalid("box lengths must be positive");
-                b.L[d] = box[d];
-                b.nc[d] = (int)(box[d] / c->P.cutoff);
-                if (b.nc[d] < 3) unsupported("fewer than 3 cells along an axis");
-                b.cs[d] = box[d] / b.nc[d];
-            }
+            for (int r = 0; r < c->nrep; r++)
+                for (int d = 0; d < 3; d++) {
+                    const double L = box[3 * r + d];
+                    if (!(L > 0)) invalid("box lengths must be positive");
+                    const int nc = (int)(L / c->P.cutoff);
+                    if (nc < 3) unsupported("fewer than 3 cells along an axis");
+                    hr[r].L[d] = L;
+                    hr[r].nc[d] = nc;
+                    hr[r].cs[d] = L / nc;
+                }
         }
         if (pos) {
             std::vector<Bead> hb(n);
             HIPCHK(hipMemcpyAsync(hb.data(), c->beads, sizeof(Bead) * n, hipMemcpyDeviceToHost, c->st));
             HIPCHK(hipStreamSynchronize(c->st));
             for (int i = 0; i < n; i++) {
+                const DevRep &R = hr[hb[i].rep];
                 for (int d = 0; d < 3; d++) {
-                    double p = pos[3 * (size_t)i + d];
-                    if (!(p >= 0 && p <= b.L[d])) invalid("particle " + std::to_string(i) + " position is out of bounds");
+                    const double p = pos[3 * (size_t)i + d];
+                    if (!(p >= 0 && p <= R.L[d])) invalid("particle " + std::to_string(i) + " position is out of bounds");
                 }
                 hb[i].x = pos[3 * (size_t)i];
                 hb[i].y = pos[3 * (size_t)i + 1];
@@ -555,8 +777,7 @@ int sm_upload_state(sm_ctx *c, const double *pos, const double *vel, const doubl
         };
         if (vel) up3(c->vel, vel);
         if (acc) up3(c->acc, acc);
-        HIPCHK(hipMemcpyAsync(c->box, &b, sizeof(DevBox), hipMemcpyHostToDevice, c->st));
-        HIPCHK(hipStreamSynchronize(c->st));
+        c->write_reps(hr);
         c->ensure_capacity(0);
     });
 }
@@ -565,7 +786,13 @@ int sm_set_temperature(sm_ctx *c, double T)
 {
     return guarded([&] {
         if (!c) invalid("null context");
-        c->P.temperature = T;
+        if (!(T >= 0)) invalid("temperature must be >= 0");
+        std::vector<DevRep> hr = c->read_reps();
+        for (auto &R : hr) {
+            R.temperature = T;
+            R.sigma = std::sqrt((6.0 * T * R.gamma) / c->P.dt);
+        }
+        c->write_reps(hr);
     });
 }
 
@@ -573,9 +800,9 @@ int sm_init_forces(sm_ctx *c)
 {
     return guarded([&] {
         if (!c) invalid("null context");
-        c->build(0);
+        c->build(0, 0);
         c->pair_force();
-        c->bonded(0, KICK_INIT, nullptr);
+        c->bonded(KICK_INIT, nullptr);
         c->check_launch();
     });
 }
@@ -593,6 +820,9 @@ int sm_step_first(sm_ctx *c, int64_t step)
 {
     return guarded([&] {
         if (!c) invalid("null context");
+        if (step < 0) invalid("negative step index");
+        // the cell table must hold this step's box move before the keys are computed
+        if (c->mc_step(step)) c->ensure_capacity(1);
         c->step_first(step);
         c->check_launch();
     });
@@ -602,11 +832,11 @@ int sm_step_finish(sm_ctx *c, int64_t step, double *measure)
 {
     return guarded([&] {
         if (!c) invalid("null context");
-        if (c->mc_step(step)) c->ensure_capacity(1);
-        c->step_finish(step, measure != nullptr);
+        const bool mc = c->mc_step(step);
+        c->step_finish(measure != nullptr, mc);
         c->check_launch();
         if (measure) {
-            HIPCHK(hipMemcpyAsync(measure, c->dres, 3 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipMemcpyAsync(measure, c->meas_dev(), 3 * sizeof(double) * c->nrep, hipMemcpyDeviceToHost, c->st));
             HIPCHK(hipStreamSynchronize(c->st));
             c->check_flags();
         }
@@ -617,15 +847,8 @@ int sm_run(sm_ctx *c, int64_t first, int32_t nsteps)
 {
     return guarded([&] {
         if (!c) invalid("null context");
-        if (nsteps < 0) invalid("negative step count");
-        long long moves = 0;
-        for (long long s = first; s < first + nsteps; s++)
-            if (c->mc_step(s)) moves++;
-        if (moves) c->ensure_capacity(moves);
-        for (long long s = first; s < first + nsteps; s++) {
-            c->step_first(s);
-            c->step_finish(s, false);
-        }
+        if (nsteps < 0 || first < 0) invalid("negative step index or count");
+        c->run(first, nsteps);
         c->check_launch();
     });
 }
@@ -634,39 +857,42 @@ int sm_mc_stats(sm_ctx *c, int64_t *trials, int64_t *accepted)
 {
     return guarded([&] {
         if (!c) invalid("null context");
-        DevBox b = c->read_box();
-        if (trials) *trials = b.trials;
-        if (accepted) *accepted = b.accepted;
-    });
-}
-
-int sm_mc_hist(sm_ctx *c, int32_t cap, double *accepted, double *rejected, int32_t *n_out)
-{
-    return guarded([&] {
-        if (!c || !n_out || cap < 0) invalid("bad argument");
-        const double dL = c->P.delta_lxy;
-        int nb = 0;
-        if (dL != 0) nb = (int)(2.0 * dL / (dL / 101.0)) + 1;
-        if (nb > 256) nb = 256;
-        *n_out = nb;
-        if (nb == 0) return;
-        DevBox b = c->read_box();
-        const int m = nb < cap ? nb : cap;
-        for (int i = 0; i < m; i++) {
-            if (accepted) accepted[i] = b.hist_accept[i];
-            if (rejected) rejected[i] = b.hist_reject[i];
+        std::vector<DevRep> hr = c->read_reps();
+        for (int r = 0; r < c->nrep; r++) {
+            if (trials) trials[r] = hr[r].trials;
+            if (accepted) accepted[r] = hr[r].accepted;
         }
     });
 }
 
-int sm_measure(sm_ctx *c, double out[3])
+int sm_mc_hist(sm_ctx *c, int32_t replica, int32_t cap, double *accepted, double *rejected, int32_t *n_out)
+{
+    return guarded([&] {
+        if (!c || !n_out || cap < 0) invalid("bad argument");
+        if (replica < 0 || replica >= c->nrep) invalid("replica index out of range");
+        std::vector<DevRep> hr = c->read_reps();
+        const DevRep &R = hr[replica];
+        const double dL = R.delta_lxy;
+        int nb = 0;
+        if (dL != 0) nb = (int)(2.0 * dL / (dL / 101.0)) + 1;
+        if (nb > kHistBins) nb = kHistBins;
+        *n_out = nb;
+        const int m = nb < cap ? nb : cap;
+        for (int i = 0; i < m; i++) {
+            if (accepted) accepted[i] = R.hist_accept[i];
+            if (rejected) rejected[i] = R.hist_reject[i];
+        }
+    });
+}
+
+int sm_measure(sm_ctx *c, double *out)
 {
     return guarded([&] {
         if (!c || !out) invalid("null argument");
-        c->build(0);
+        c->build(0, kStepKeep);
         c->measure_device();
         c->check_launch();
-        HIPCHK(hipMemcpyAsync(out, c->dres, 3 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipMemcpyAsync(out, c->meas_dev(), 3 * sizeof(double) * c->nrep, hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
         c->check_flags();
     });
@@ -676,23 +902,12 @@ int sm_compute_forces(sm_ctx *c, double *pair_out, double *chain_out)
 {
     return guarded([&] {
         if (!c) invalid("null context");
-        const int n = c->n;
-        c->build(0);
+        c->build(0, kStepKeep);
         c->pair_force();
-        c->bonded(0, KICK_NONE, c->fscratch);
+        c->bonded(KICK_NONE, c->fscratch);
         c->check_launch();
-        std::vector<float4> h(n);
-        auto out3 = [&](const float4 *src, double *dst) {
-            HIPCHK(hipMemcpyAsync(h.data(), src, sizeof(float4) * n, hipMemcpyDeviceToHost, c->st));
-            HIPCHK(hipStreamSynchronize(c->st));
-            for (int i = 0; i < n; i++) {
-                dst[3 * (size_t)i] = h[i].x;
-                dst[3 * (size_t)i + 1] = h[i].y;
-                dst[3 * (size_t)i + 2] = h[i].z;
-            }
-        };
-        if (pair_out) out3(c->fpair, pair_out);
-        if (chain_out) out3(c->fscratch, chain_out);
+        if (pair_out) download_f4(c, c->fpair, pair_out);
+        if (chain_out) download_f4(c, c->fscratch, chain_out);
         c->check_flags();
     });
 }
@@ -701,10 +916,14 @@ int sm_cell_keys(sm_ctx *c, int32_t *keys)
 {
     return guarded([&] {
         if (!c || !keys) invalid("null argument");
-        c->first_and_keys(0);
+        c->first_and_keys(0, kStepKeep);
         c->check_launch();
-        HIPCHK(hipMemcpyAsync(keys, c->k0, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->st));
+        std::vector<uint32_t> h(c->n);
+        HIPCHK(hipMemcpyAsync(h.data(), c->k0, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
+        for (int r = 0; r < c->nrep; r++)
+            for (int i = c->rh[r].first; i < c->rh[r].first + c->rh[r].count; i++)
+                keys[i] = (int32_t)((long long)h[i] - c->rh[r].cellbase);
         c->check_flags();
     });
 }
@@ -713,9 +932,9 @@ int sm_pair_count(sm_ctx *c, int32_t *counts, int64_t *total)
 {
     return guarded([&] {
         if (!c) invalid("null context");
-        c->build(0);
-        launch_pair_gather(c->st, c->P, c->chunk, GATHER_COUNT, c->sbeads, c->skey(), c->sidx(), c->cstart, c->cend, c->box,
-                           c->uconstD, c->partA, c->dcounts, nullptr, nullptr, 0);
+        c->build(0, kStepKeep);
+        launch_pair_gather(c->st, c->P, c->chunk, GATHER_COUNT, c->sbeads, c->skey(), c->sidx(), c->cstart, c->cend,
+                           c->reps, c->uconstD, nullptr, c->dcounts, nullptr, nullptr, 0);
         c->check_launch();
         std::vector<int32_t> h(c->n);
         HIPCHK(hipMemcpyAsync(h.data(), c->dcounts, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->st));
@@ -737,10 +956,10 @@ int sm_pair_list(sm_ctx *c, int32_t *pairs, int64_t cap, int64_t *n_out)
             c->dpairs = dalloc<int>(2 * (size_t)cap);
             c->pairs_cap = cap;
         }
-        c->build(0);
+        c->build(0, kStepKeep);
         HIPCHK(hipMemsetAsync(c->dnp, 0, sizeof(unsigned long long), c->st));
-        launch_pair_gather(c->st, c->P, c->chunk, GATHER_LIST, c->sbeads, c->skey(), c->sidx(), c->cstart, c->cend, c->box,
-                           c->uconstD, c->partA, c->dcounts, c->dpairs, c->dnp, cap);
+        launch_pair_gather(c->st, c->P, c->chunk, GATHER_LIST, c->sbeads, c->skey(), c->sidx(), c->cstart, c->cend,
+                           c->reps, c->uconstD, nullptr, c->dcounts, c->dpairs, c->dnp, cap);
         c->check_launch();
         unsigned long long np = 0;
         HIPCHK(hipMemcpyAsync(&np, c->dnp, sizeof np, hipMemcpyDeviceToHost, c->st));
@@ -755,21 +974,24 @@ int sm_pair_list(sm_ctx *c, int32_t *pairs, int64_t cap, int64_t *n_out)
     });
 }
 
-int sm_potential_delta(sm_ctx *c, const double scale[3], double *du_pair, double *du_chain)
+int sm_potential_delta(sm_ctx *c, const double *scale, double *du_pair, double *du_chain)
 {
     return guarded([&] {
         if (!c || !scale) invalid("null argument");
-        HIPCHK(hipMemcpyAsync(&c->box->scale, scale, 3 * sizeof(double), hipMemcpyHostToDevice, c->st));
-        c->build(0);
+        std::vector<DevRep> hr = c->read_reps();
+        for (int r = 0; r < c->nrep; r++)
+            for (int d = 0; d < 3; d++) hr[r].scale[d] = scale[3 * r + d];
+        c->write_reps(hr);
+        c->build(0, kStepKeep);
         c->delta_device();
-        launch_reduce_sum(c->st, c->partA, cells_blocks(c->n, c->chunk), c->dres + 0);
-        launch_reduce_sum(c->st, c->partB, chain_blocks(c->nchains), c->dres + 1);
         c->check_launch();
-        double r[2];
-        HIPCHK(hipMemcpyAsync(r, c->dres, 2 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+        std::vector<double> h(2 * (size_t)c->nrep);
+        HIPCHK(hipMemcpyAsync(h.data(), c->du_dev(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
-        if (du_pair) *du_pair = r[0];
-        if (du_chain) *du_chain = r[1];
+        for (int r = 0; r < c->nrep; r++) {
+            if (du_pair) du_pair[r] = h[2 * r];
+            if (du_chain) du_chain[r] = h[2 * r + 1];
+        }
         c->check_flags();
     });
 }
@@ -784,9 +1006,378 @@ int sm_tune(sm_ctx *c, const char *knob, int32_t value)
         } else if (std::strcmp(knob, "pair_variant") == 0) {
             if (value < 0 || value > 1) invalid("pair_variant must be 0 or 1");
             c->pair_variant = value;
+        } else if (std::strcmp(knob, "graph") == 0) {
+            if (value < 0 || value > 1) invalid("graph must be 0 or 1");
+            c->use_graph = value != 0;
         } else {
             invalid(std::string("unknown knob ") + knob);
         }
+        c->drop_graph();
+    });
+}
+
+int sm_tune_get(sm_ctx *c, const char *knob, int32_t *value)
+{
+    return guarded([&] {
+        if (!c || !knob || !value) invalid("null argument");
+        if (std::strcmp(knob, "pair_chunk") == 0) *value = c->chunk;
+        else if (std::strcmp(knob, "pair_variant") == 0) *value = c->pair_variant;
+        else if (std::strcmp(knob, "graph") == 0) *value = c->use_graph ? 1 : 0;
+        else invalid(std::string("unknown knob ") + knob);
+    });
+}
+
+int sm_autotune(sm_ctx *c, int32_t steps, double *best_ms_per_step)
+{
+    return guarded([&] {
+        if (!c) invalid("null context");
+        if (steps < 8 || steps > 100000) invalid("steps per trial must be in [8, 100000]");
+        if (c->prof) invalid("profiling is on");
+        const long long s0 = c->read_glob().step + 1;
+        // every trial starts from the same state; reserve cells for the trials' moves
+        // first so that no trial moves the cell bases
+        long long moves = 0;
+        for (long long s = s0; s < s0 + steps; s++)
+            if (c->mc_step(s)) moves++;
+        c->ensure_capacity(moves);
+        const int n = c->n;
+        Bead *b0 = dalloc<Bead>(n);
+        float4 *v0 = dalloc<float4>(n), *a0 = dalloc<float4>(n);
+        DevRep *r0 = dalloc<DevRep>(c->nrep);
+        DevGlobal *g0 = dalloc<DevGlobal>(1);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        auto cleanup = [&] {
+            dfree(b0); dfree(v0); dfree(a0); dfree(r0); dfree(g0);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        };
+        try {
+            hipStream_t st = c->st;
+            auto copy = [&](void *d, const void *s, size_t bytes) {
+                HIPCHK(hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, st));
+            };
+            auto save = [&] {
+                copy(b0, c->beads, sizeof(Bead) * n); copy(v0, c->vel, sizeof(float4) * n);
+                copy(a0, c->acc, sizeof(float4) * n); copy(r0, c->reps, sizeof(DevRep) * c->nrep);
+                copy(g0, c->glob, sizeof(DevGlobal));
+            };
+            auto restore = [&] {
+                copy(c->beads, b0, sizeof(Bead) * n); copy(c->vel, v0, sizeof(float4) * n);
+                copy(c->acc, a0, sizeof(float4) * n); copy(c->reps, r0, sizeof(DevRep) * c->nrep);
+                copy(c->glob, g0, sizeof(DevGlobal));
+            };
+            HIPCHK(hipEventCreate(&e0));
+            HIPCHK(hipEventCreate(&e1));
+            save();
+            // launch shapes that leave every result bit-identical: the chunk of the one-phase
+            // pair kernel (a cell is processed the same way whatever chunk holds it) and the
+            // graph replay (the same kernels on the same stream)
+            struct Cand { int chunk, variant, graph; };
+            std::vector<Cand> cands;
+            for (int ch : {4, 8, 16, 32, 64}) cands.push_back({ch, 0, 1});
+            double best = 1e300;
+            Cand bestc{c->chunk, c->pair_variant, c->use_graph ? 1 : 0};
+            auto trial = [&](const Cand &k) {
+                c->chunk = k.chunk;
+                c->pair_variant = k.variant;
+                c->use_graph = k.graph != 0;
+                c->drop_graph();
+                restore();
+                c->run(s0, 8);          // warm: first launches, graph capture
+                restore();
+                HIPCHK(hipEventRecord(e0, st));
+                c->run(s0, steps);
+                HIPCHK(hipEventRecord(e1, st));
+                HIPCHK(hipEventSynchronize(e1));
+                c->check_launch();
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                const double per = ms / steps;
+                if (per < best) {
+                    best = per;
+                    bestc = k;
+                }
+            };
+            for (const auto &k : cands) trial(k);
+            trial({bestc.chunk, bestc.variant, 0});   // is the graph worth it here
+            c->chunk = bestc.chunk;
+            c->pair_variant = bestc.variant;
+            c->use_graph = bestc.graph != 0;
+            c->drop_graph();
+            restore();
+            HIPCHK(hipStreamSynchronize(st));
+            c->check_flags();
+            if (best_ms_per_step) *best_ms_per_step = best;
+        } catch (...) {
+            (void)hipStreamSynchronize(c->st);
+            cleanup();
+            throw;
+        }
+        cleanup();
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// interaction constants as adjoint parameters
+// ---------------------------------------------------------------------------------------
+namespace {
+
+void check_pair(const sm_ctx *c, int a, int b)
+{
+    if (a < 0 || b < 0 || a >= c->P.nT || b >= c->P.nT) invalid("type index out of range");
+}
+
+// laradjiRevaleeFC / laradjiRevaleePC rows (potentials/laradjiRevalee.h:84-152) of the
+// type pair (a, b) and (b, a) from Umin, Umax, keeping each row's r_m and r_c
+void set_pair_rows(sm_ctx *c, int a, int b, double umin, double umax)
+{
+    const int nT = c->P.nT;
+    for (int pass = 0; pass < 2; pass++) {
+        const size_t row = 6 * (size_t)(pass ? b * nT + a : a * nT + b);
+        double *F = &c->hfconst[row], *U = &c->huconst[row];
+        const double rm = F[0], rc = F[3];
+        F[1] = (2.0 * (umax - umin)) / (rm * rm);
+        F[2] = 0;
+        F[4] = (6.0 * umin) / ((rc - rm) * (rc - rm));
+        F[5] = (6.0 * umin) / ((rc - rm) * (rc - rm) * (rc - rm));
+        U[0] = rm;
+        U[1] = (umax - umin) / (rm * rm);
+        U[2] = umin;
+        U[3] = rc;
+        U[4] = (3.0 * umin) / ((rc - rm) * (rc - rm));
+        U[5] = (2.0 * umin) / ((rc - rm) * (rc - rm) * (rc - rm));
+    }
+}
+
+void upload_constants(sm_ctx *c)
+{
+    const size_t nt = c->hfconst.size();
+    std::vector<float> fF(nt);
+    for (size_t k = 0; k < nt; k++) fF[k] = (float)c->hfconst[k];
+    HIPCHK(hipMemcpyAsync(c->fconstF, fF.data(), sizeof(float) * nt, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->uconstD, c->huconst.data(), sizeof(double) * nt, hipMemcpyHostToDevice, c->st));
+    if (!c->hcconst.empty())
+        HIPCHK(hipMemcpyAsync(c->cconstD, c->hcconst.data(), sizeof(double4) * c->hcconst.size(),
+                              hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+}
+
+// FP64 F table and the d/d(Umin), d/d(Umax) tables of the (a, b) rows
+void upload_adjoint_tables(sm_ctx *c, int a, int b)
+{
+    const int nT = c->P.nT;
+    const size_t nt = 6 * (size_t)nT * nT;
+    std::vector<double> dF(2 * nt, 0.0), dU(2 * nt, 0.0);
+    for (int pass = 0; pass < 2; pass++) {
+        const size_t row = 6 * (size_t)(pass ? b * nT + a : a * nT + b);
+        const double rm = c->hfconst[row], rc = c->hfconst[row + 3];
+        const double w = rc - rm;
+        double *Fmin = &dF[row], *Fmax = &dF[nt + row], *Umin = &dU[row], *Umax = &dU[nt + row];
+        Fmin[1] = -2.0 / (rm * rm);
+        Fmin[4] = 6.0 / (w * w);
+        Fmin[5] = 6.0 / (w * w * w);
+        Fmax[1] = 2.0 / (rm * rm);
+        Umin[1] = -1.0 / (rm * rm);
+        Umin[2] = 1.0;
+        Umin[4] = 3.0 / (w * w);
+        Umin[5] = 2.0 / (w * w * w);
+        Umax[1] = 1.0 / (rm * rm);
+    }
+    if (!c->adjF) {
+        c->adjF = dalloc<double>(nt);
+        c->adjDF = dalloc<double>(2 * nt);
+        c->adjDU = dalloc<double>(2 * nt);
+    }
+    HIPCHK(hipMemcpyAsync(c->adjF, c->hfconst.data(), sizeof(double) * nt, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->adjDF, dF.data(), sizeof(double) * 2 * nt, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->adjDU, dU.data(), sizeof(double) * 2 * nt, hipMemcpyHostToDevice, c->st));
+}
+
+// device copy of the evolving state
+struct StateCopy {
+    Bead *beads = nullptr;
+    float4 *vel = nullptr, *acc = nullptr;
+    DevRep *reps = nullptr;
+    DevGlobal *glob = nullptr;
+    void alloc(const sm_ctx *c)
+    {
+        beads = dalloc<Bead>(c->n);
+        vel = dalloc<float4>(c->n);
+        acc = dalloc<float4>(c->n);
+        reps = dalloc<DevRep>(c->nrep);
+        glob = dalloc<DevGlobal>(1);
+    }
+    void release() { dfree(beads); dfree(vel); dfree(acc); dfree(reps); dfree(glob); }
+    void save(const sm_ctx *c)
+    {
+        HIPCHK(hipMemcpyAsync(beads, c->beads, sizeof(Bead) * c->n, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(vel, c->vel, sizeof(float4) * c->n, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(acc, c->acc, sizeof(float4) * c->n, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(reps, c->reps, sizeof(DevRep) * c->nrep, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(glob, c->glob, sizeof(DevGlobal), hipMemcpyDeviceToDevice, c->st));
+    }
+    void load(sm_ctx *c) const
+    {
+        HIPCHK(hipMemcpyAsync(c->beads, beads, sizeof(Bead) * c->n, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->vel, vel, sizeof(float4) * c->n, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->acc, acc, sizeof(float4) * c->n, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->reps, reps, sizeof(DevRep) * c->nrep, hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->glob, glob, sizeof(DevGlobal), hipMemcpyDeviceToDevice, c->st));
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int sm_set_params(sm_ctx *c, int32_t type_a, int32_t type_b, const double *theta)
+{
+    return guarded([&] {
+        if (!c || !theta) invalid("null argument");
+        check_pair(c, type_a, type_b);
+        set_pair_rows(c, type_a, type_b, theta[0], theta[1]);
+        for (auto &k : c->hcconst) {
+            k.y = theta[2];
+            k.w = theta[3];
+        }
+        upload_constants(c);
+    });
+}
+
+int sm_get_params(sm_ctx *c, int32_t type_a, int32_t type_b, double *theta)
+{
+    return guarded([&] {
+        if (!c || !theta) invalid("null argument");
+        check_pair(c, type_a, type_b);
+        const double *F = &c->hfconst[6 * (size_t)(type_a * c->P.nT + type_b)];
+        const double rm = F[0], rc = F[3];
+        const double umin = F[4] * (rc - rm) * (rc - rm) / 6.0;
+        theta[0] = umin;
+        theta[1] = F[1] * rm * rm / 2.0 + umin;
+        theta[2] = c->hcconst.empty() ? 0.0 : c->hcconst[0].y;
+        theta[3] = c->hcconst.empty() ? 0.0 : c->hcconst[0].w;
+    });
+}
+
+int sm_adjoint(sm_ctx *c, int32_t type_a, int32_t type_b, int32_t nsteps, int32_t checkpoint, double *J,
+               double *grad)
+{
+    return guarded([&] {
+        if (!c || !J) invalid("null argument");
+        if (c->nrep != 1) unsupported("the adjoint runs on single-system contexts");
+        if (nsteps < 1 || checkpoint < 1 || checkpoint > nsteps) invalid("need 1 <= checkpoint <= nsteps");
+        check_pair(c, type_a, type_b);
+        const int n = c->n, S = nsteps, k = checkpoint;
+        const double obj = 1.0 / S;
+        long long moves = 0;
+        for (long long s = 1; s <= S; s++)
+            if (c->mc_step(s)) moves++;
+        c->ensure_capacity(moves);
+        upload_adjoint_tables(c, type_a, type_b);
+
+        const int ncp = (S + k - 1) / k;   // checkpoints at steps 0, k, 2k, ...
+        StateCopy entry;
+        std::vector<StateCopy> cps(ncp);
+        Bead *segBeads = nullptr;
+        DevRep *segPre = nullptr, *segPost = nullptr;
+        double4 *X = nullptr, *V = nullptr, *A = nullptr, *F = nullptr, *HV = nullptr;
+        double *g = nullptr, *gs = nullptr;
+        auto cleanup = [&] {
+            entry.release();
+            for (auto &cp : cps) cp.release();
+            dfree(segBeads); dfree(segPre); dfree(segPost);
+            dfree(X); dfree(V); dfree(A); dfree(F); dfree(HV); dfree(g); dfree(gs);
+        };
+        try {
+            hipStream_t st = c->st;
+            entry.alloc(c);
+            entry.save(c);
+            for (auto &cp : cps) cp.alloc(c);
+
+            // forward: init forces, then steps 1..S, J = mean of U after each kick
+            c->build(0, 0);
+            c->pair_force();
+            c->bonded(KICK_INIT, nullptr);
+            cps[0].save(c);
+            double Jsum = 0.0;
+            for (int s = 1; s <= S; s++) {
+                c->step_first(s);
+                c->step_finish(true, c->mc_step(s));
+                double m[3];
+                HIPCHK(hipMemcpyAsync(m, c->meas_dev(), sizeof m, hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+                Jsum += m[0] + m[1];
+                if (s % k == 0 && s / k < ncp) cps[s / k].save(c);
+            }
+            c->check_launch();
+            c->check_flags();
+            *J = Jsum * obj;
+
+            if (grad) {
+                segBeads = dalloc<Bead>((size_t)k * n);
+                segPre = dalloc<DevRep>((size_t)k * c->nrep);
+                segPost = dalloc<DevRep>((size_t)k * c->nrep);
+                X = dalloc<double4>(n); V = dalloc<double4>(n); A = dalloc<double4>(n);
+                F = dalloc<double4>(n); HV = dalloc<double4>(n);
+                g = dalloc<double>(4 * (size_t)n);
+                gs = dalloc<double>(4);
+                for (double4 *p : {X, V, A, F, HV}) HIPCHK(hipMemsetAsync(p, 0, sizeof(double4) * n, st));
+                HIPCHK(hipMemsetAsync(g, 0, sizeof(double) * 4 * n, st));
+                const AdjTables T{c->adjF, c->adjDF, c->adjDU};
+                auto adjoint_forces = [&](double o, int hv) {
+                    launch_adj_pair(st, c->P, c->sbeads, c->skey(), c->sidx(), c->cstart, c->cend, c->reps, T, A, F,
+                                    HV, g, o, hv);
+                    launch_adj_chain(st, c->P, c->beads, c->chainPerBead, c->cconstD, c->reps, A, F, HV, g, o, hv);
+                };
+                for (int seg = ncp - 1; seg >= 0; seg--) {
+                    const int s0 = seg * k, s1 = std::min(S, s0 + k);
+                    // recompute the segment, keeping each step's positions before its MC move
+                    cps[seg].load(c);
+                    for (int s = s0 + 1; s <= s1; s++) {
+                        const int t = s - s0 - 1;
+                        c->step_first(s);
+                        c->step_finish(false, false);
+                        HIPCHK(hipMemcpyAsync(segBeads + (size_t)t * n, c->beads, sizeof(Bead) * n,
+                                              hipMemcpyDeviceToDevice, st));
+                        HIPCHK(hipMemcpyAsync(segPre + (size_t)t * c->nrep, c->reps, sizeof(DevRep) * c->nrep,
+                                              hipMemcpyDeviceToDevice, st));
+                        if (c->mc_step(s)) c->mc_move();
+                        HIPCHK(hipMemcpyAsync(segPost + (size_t)t * c->nrep, c->reps, sizeof(DevRep) * c->nrep,
+                                              hipMemcpyDeviceToDevice, st));
+                    }
+                    for (int s = s1; s > s0; s--) {
+                        const int t = s - s0 - 1;
+                        HIPCHK(hipMemcpyAsync(c->beads, segBeads + (size_t)t * n, sizeof(Bead) * n,
+                                              hipMemcpyDeviceToDevice, st));
+                        HIPCHK(hipMemcpyAsync(c->reps, segPre + (size_t)t * c->nrep, sizeof(DevRep) * c->nrep,
+                                              hipMemcpyDeviceToDevice, st));
+                        c->build(0, kStepKeep);
+                        adjoint_forces(obj, 1);
+                        launch_adj_step(st, c->P, c->beads, segPost + (size_t)t * c->nrep, c->mc_step(s) ? 1 : 0,
+                                        X, V, A, F, HV, obj, s - 1 >= 1 ? 1 : 0);
+                    }
+                }
+                // a_0 = F(x_0) + ...: its adjoint meets dF/dtheta at the initial positions
+                cps[0].load(c);
+                c->build(0, kStepKeep);
+                adjoint_forces(0.0, 0);
+                launch_seg_sum(st, c->segB, g, gs + 0, 1);
+                for (int p = 1; p < kNTheta; p++) launch_seg_sum(st, c->segB, g + (size_t)p * n, gs + p, 1);
+                c->check_launch();
+                HIPCHK(hipMemcpyAsync(grad, gs, sizeof(double) * kNTheta, hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+            }
+            entry.load(c);
+            HIPCHK(hipStreamSynchronize(st));
+            c->drop_graph();
+        } catch (...) {
+            (void)hipStreamSynchronize(c->st);
+            cleanup();
+            throw;
+        }
+        cleanup();
     });
 }
 
@@ -812,224 +1403,6 @@ int sm_profile_read(sm_ctx *c, int32_t cap, const char **names, double *avg_ms, 
             if (count) count[m] = c->prof_cnt[k];
         }
         if (n_out) *n_out = m;
-    });
-}
-
-// ---------------------------------------------------------------------------------------
-// .mpd and generators
-// ---------------------------------------------------------------------------------------
-}  // extern "C"
-
-struct sm_mpd {
-    smio::System sys;
-    std::vector<int32_t> chain;
-    std::vector<double> chain_const;
-};
-
-namespace {
-
-const struct { const char *name; smio::Cmd cmd; } kScalars[] = {
-    {"gamma", smio::GAMMA}, {"initialTemp", smio::INITIALTEMP}, {"finalTemp", smio::FINALTEMP},
-    {"seed", smio::SEED}, {"cutoff", smio::CUTOFF}, {"initialTime", smio::INITIALTIME},
-    {"finalTime", smio::FINALTIME}, {"deltaT", smio::DELTAT}, {"storeInterval", smio::STOREINTERVAL},
-    {"measureInterval", smio::MEASUREINTERVAL}, {"deltaLXY", smio::DELTALXY},
-    {"removeSolvent", smio::REMOVESOLVENT}, {"tempStepInterval", smio::TEMPSTEPINTERVAL},
-    {"tension", smio::TENSION}, {"nTypes", smio::NTYPES}, {"nParticles", smio::NPARTICLES},
-    {"nMolecules", smio::NMOLECULES}};
-
-double *scalar_ref(smio::System &s, smio::Cmd c)
-{
-    switch (c) {
-    case smio::GAMMA: return &s.gamma;
-    case smio::INITIALTEMP: return &s.initialTemp;
-    case smio::FINALTEMP: return &s.finalTemp;
-    case smio::CUTOFF: return &s.cutoff;
-    case smio::INITIALTIME: return &s.initialTime;
-    case smio::FINALTIME: return &s.finalTime;
-    case smio::DELTAT: return &s.deltaT;
-    case smio::STOREINTERVAL: return &s.storeInterval;
-    case smio::MEASUREINTERVAL: return &s.measureInterval;
-    case smio::DELTALXY: return &s.deltaLXY;
-    case smio::REMOVESOLVENT: return &s.removeSolvent;
-    case smio::TEMPSTEPINTERVAL: return &s.tempStepInterval;
-    case smio::TENSION: return &s.tension;
-    default: return nullptr;
-    }
-}
-
-int io_guard(const std::function<void()> &f)
-{
-    try {
-        f();
-        return SM_OK;
-    } catch (const ApiError &e) {
-        g_err = e.what();
-        return e.code;
-    } catch (const std::exception &e) {
-        g_err = e.what();
-        return SM_ERR_IO;
-    }
-}
-
-}  // namespace
-
-extern "C" {
-
-int sm_mpd_read(const char *name, sm_mpd **out)
-{
-    if (!name || !out) {
-        g_err = "null argument";
-        return SM_ERR_INVALID;
-    }
-    *out = nullptr;
-    return io_guard([&] {
-        auto *m = new sm_mpd();
-        try {
-            m->sys = smio::read_file(std::string(name) + ".mpd");
-        } catch (...) {
-            delete m;
-            throw;
-        }
-        *out = m;
-    });
-}
-
-int sm_mpd_write(const sm_mpd *m, const char *name)
-{
-    if (!m || !name) {
-        g_err = "null argument";
-        return SM_ERR_INVALID;
-    }
-    return io_guard([&] { smio::write_file(m->sys, std::string(name) + ".mpd"); });
-}
-
-int sm_mpd_free(sm_mpd *m)
-{
-    delete m;
-    return SM_OK;
-}
-
-int sm_mpd_system(const sm_mpd *mc, sm_system *out)
-{
-    if (!mc || !out) {
-        g_err = "null argument";
-        return SM_ERR_INVALID;
-    }
-    sm_mpd *m = const_cast<sm_mpd *>(mc);
-    return guarded([&] {
-        const smio::System &s = m->sys;
-        m->chain.clear();
-        m->chain_const.clear();
-        for (const auto &mo : s.mol) {
-            if (mo.type != smio::CHAIN)
-                unsupported("molecule type " + std::to_string(mo.type) + " is not implemented (CHAIN only)");
-            for (const auto &b : mo.bonds) {
-                m->chain.insert(m->chain.end(), {b[0], b[1], b[2]});
-                m->chain_const.insert(m->chain_const.end(), mo.constants.begin(), mo.constants.end());
-            }
-        }
-        if (s.gamma <= 0) unsupported("gamma must be > 0 (per-type gammaType is not implemented)");
-        std::memset(out, 0, sizeof *out);
-        out->n_particles = s.nParticles;
-        out->n_types = s.nTypes;
-        out->pos = s.pos.data();
-        out->type = s.type.data();
-        out->vel = s.vel.data();
-        out->acc = nullptr;
-        out->n_chain_blocks = (int32_t)(m->chain.size() / 3);
-        out->chain = m->chain.data();
-        out->chain_const = m->chain_const.data();
-        out->fconst = s.fconst.data();
-        out->uconst = s.uconst.data();
-        for (int d = 0; d < 3; d++) out->box[d] = s.size[d];
-        out->cutoff = s.cutoff;
-        out->dt = s.deltaT;
-        out->gamma = s.gamma;
-        out->temperature = s.initialTemp;
-        out->delta_lxy = s.deltaLXY;
-        out->tension = s.tension;
-        out->seed = (uint64_t)(uint32_t)s.seed;
-    });
-}
-
-int sm_mpd_get(const sm_mpd *m, const char *key, double *value)
-{
-    if (!m || !key || !value) {
-        g_err = "null argument";
-        return SM_ERR_INVALID;
-    }
-    for (const auto &k : kScalars) {
-        if (std::strcmp(k.name, key) != 0) continue;
-        smio::System &s = const_cast<sm_mpd *>(m)->sys;
-        switch (k.cmd) {
-        case smio::SEED: *value = s.seed; return SM_OK;
-        case smio::NTYPES: *value = s.nTypes; return SM_OK;
-        case smio::NPARTICLES: *value = s.nParticles; return SM_OK;
-        case smio::NMOLECULES: *value = s.nMolecules; return SM_OK;
-        default: *value = *scalar_ref(s, k.cmd); return SM_OK;
-        }
-    }
-    g_err = std::string("unknown scalar key ") + key;
-    return SM_ERR_INVALID;
-}
-
-int sm_mpd_set(sm_mpd *m, const char *key, double value)
-{
-    if (!m || !key) {
-        g_err = "null argument";
-        return SM_ERR_INVALID;
-    }
-    for (const auto &k : kScalars) {
-        if (std::strcmp(k.name, key) != 0) continue;
-        double *p = scalar_ref(m->sys, k.cmd);
-        if (k.cmd == smio::SEED) m->sys.seed = (int)value;
-        else if (!p) {
-            g_err = std::string(key) + " is not settable";
-            return SM_ERR_INVALID;
-        } else *p = value;
-        m->sys.present[k.cmd] = true;
-        return SM_OK;
-    }
-    g_err = std::string("unknown scalar key ") + key;
-    return SM_ERR_INVALID;
-}
-
-int sm_mpd_set_state(sm_mpd *m, const double *pos, const double *vel, const double box[3])
-{
-    if (!m) {
-        g_err = "null argument";
-        return SM_ERR_INVALID;
-    }
-    const size_t n3 = 3 * (size_t)m->sys.nParticles;
-    if (pos) std::memcpy(m->sys.pos.data(), pos, n3 * sizeof(double));
-    if (vel) std::memcpy(m->sys.vel.data(), vel, n3 * sizeof(double));
-    if (box) for (int d = 0; d < 3; d++) m->sys.size[d] = box[d];
-    return SM_OK;
-}
-
-int sm_gen_liposome(int32_t seed, int32_t n_lipids, double rho, sm_mpd **out)
-{
-    if (!out || n_lipids <= 0 || !(rho > 0)) {
-        g_err = "bad argument";
-        return SM_ERR_INVALID;
-    }
-    return io_guard([&] {
-        auto *m = new sm_mpd();
-        m->sys = smio::gen_liposome(seed, n_lipids, rho);
-        *out = m;
-    });
-}
-
-int sm_gen_bilayer(int32_t seed, int32_t n_lipids, double rho, sm_mpd **out)
-{
-    if (!out || n_lipids <= 0 || !(rho > 0)) {
-        g_err = "bad argument";
-        return SM_ERR_INVALID;
-    }
-    return io_guard([&] {
-        auto *m = new sm_mpd();
-        m->sys = smio::gen_bilayer(seed, n_lipids, rho);
-        *out = m;
     });
 }
 

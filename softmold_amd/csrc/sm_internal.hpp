@@ -1,16 +1,22 @@
 // Shared device data layout and kernel launchers of the MI355X MD-step solver.
 //
-// HBM layout (one system, N beads):
-//   beads   Bead[N]      FP64 position + type, original (molecule) order      32 B/bead
-//   vel     float4[N]    FP32 velocity                                        16 B/bead
-//   acc     float4[N]    FP32 acceleration a (= force, unit mass)             16 B/bead
-//   fpair   float4[N]    FP32 pair force of the current step                   16 B/bead
-//   key/idx uint32[N]x2  cell key and particle index, sorted by key (ping-pong)
-//   sbeads  Bead[N]      positions in sorted (cell) order                      32 B/bead
-//   cstart/cend int32[C] dense cell table over C = nx*ny*nz cells, never cleared:
-//                        an entry is valid only when skey[cstart[k]] == k, so
-//                        stale entries of emptied cells are rejected without a reset.
+// A context holds R >= 1 independent replica systems that share one set of interaction
+// constants, cutoff and time step (a single system is R = 1).  Replica r owns the bead
+// range [first_r, first_r + count_r) in molecule order; because the cell key of replica r
+// is offset by cellbase_r, the same range is its slot range in sorted (cell) order.
+//
+// HBM layout (N = total beads):
+//   beads   Bead[N]      FP64 position, type, replica index; molecule order       32 B/bead
+//   vel     float4[N]    FP32 velocity                                             16 B/bead
+//   acc     float4[N]    FP32 acceleration a (= force, unit mass)                  16 B/bead
+//   fpair   float4[N]    FP32 pair force of the current step                        16 B/bead
+//   key/idx uint32[N]x2  global cell key and particle index, sorted by key (ping-pong)
+//   sbeads  Bead[N]      positions in sorted (cell) order                           32 B/bead
+//   cstart/cend int32[C] dense cell table over C = sum_r capacity_r cells, never cleared:
+//                        an entry is valid only when skey[cstart[k]] == k, so stale
+//                        entries of emptied cells are rejected without a reset.
 //   chain   int2[N]      per bead: first bead of its CHAIN (or -1), length | const id << 16
+//   reps    DevRep[R]    box, cell grid, thermostat and MC state of each replica
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -19,20 +25,26 @@ namespace smdev {
 
 constexpr int kMaxTypes = 8;           // LDS constant table bound (6 * 8 * 8 floats)
 constexpr int kWave = 64;
+constexpr int kHistBins = 256;
 
 struct __align__(32) Bead {
     double x, y, z;
     int type;
-    int pad;
+    int rep;      // replica index
 };
 
-// Box and MC state, device resident so a captured step needs no host decision.
-struct DevBox {
+// One replica system, device resident, so a captured step needs no host decision.
+struct DevRep {
     double L[3];
     double cs[3];
     int nc[3];
-    int flags;                 // bit0 non-finite/out-of-box position, bit1 cell table overflow
-    long long capacity;        // cells the dense table holds
+    int first, count;          // bead range (molecule order == sorted order)
+    int cfirst, ccount;        // chain range
+    int pad0;
+    long long cellbase;        // first global cell key
+    long long capacity;        // cells reserved for this replica
+    unsigned long long seed;   // counter-RNG key
+    double temperature, gamma, sigma, delta_lxy, tension;
     // MC trial state
     double scale[3];
     double newL[3];
@@ -40,67 +52,86 @@ struct DevBox {
     double u_accept;
     double dU_pair, dU_chain;
     int accept;
-    int pad2;
+    int pad1;
     long long trials, accepted;
-    double hist_accept[256];
-    double hist_reject[256];
-    // measurement results {U_pair, U_chain, K}
-    double meas[3];
+    double hist_accept[kHistBins];
+    double hist_reject[kHistBins];
+};
+
+// Context-wide device state.
+struct DevGlobal {
+    int flags;
+    int pad;
+    long long step;            // step index of the step in progress (Philox counter)
 };
 
 enum : int { FLAG_BAD_POSITION = 1, FLAG_CELL_OVERFLOW = 2, FLAG_SORT_TIMEOUT = 4 };
 
 struct Params {
-    int n;
+    int n;          // beads, all replicas
+    int nrep;
     int nT;
     double cutoff;
     double dt;
-    double gamma;
-    double temperature;
-    double delta_lxy;
-    double tension;
-    unsigned long long seed;
 };
 
 // Counter-RNG stream tags (word 3 of the Philox counter).
 enum : unsigned { TAG_LANGEVIN = 0, TAG_MC = 1, TAG_LANGEVIN_INIT = 2 };
 
+// Deterministic segmented FP64 sums: a segment set is a list of ranges (one per replica)
+// cut into tiles of kSegTile elements that start at the range start, so a replica's sum
+// does not depend on the other replicas.
+constexpr int kSegTile = 2048;
+struct SegTiles {
+    int2 *tile;        // (begin, end) per tile, device
+    int *rep_first;    // first tile of each replica (R + 1 entries), device
+    int ntiles;
+    int nrep;
+    double *tilesum;   // ntiles, device
+};
+// out[r * stride] = sum of x over replica r's range
+void launch_seg_sum(hipStream_t st, const SegTiles &S, const double *x, double *out, int stride);
+
 // ---- launchers (sm_kernels.hip) ---------------------------------------------------------
-void launch_first(hipStream_t st, const Params &P, Bead *beads, float4 *vel, const float4 *acc,
-                  const DevBox *box, uint32_t *key, uint32_t *idx, int integrate);
+// step_set >= 0: the device step counter is set to step_set; < 0: it is advanced by one
+// (the form a captured graph replays).
+void launch_first(hipStream_t st, const Params &P, Bead *beads, float4 *vel, const float4 *acc, const DevRep *reps,
+                  DevGlobal *glob, uint32_t *key, uint32_t *idx, int integrate, long long step_set);
 void launch_reorder(hipStream_t st, int n, const uint32_t *skey, const uint32_t *sidx, const Bead *beads,
                     Bead *sbeads, int *cstart, int *cend);
 // variant 0: one phase (FP64 test, force law under the branch); 1: FP32 screen into a
 // per-lane hit list, then FP64 test + force law on the hits
 void launch_pair_force(hipStream_t st, const Params &P, int chunk, int variant, const Bead *sbeads,
                        const uint32_t *skey, const uint32_t *sidx, const int *cstart, const int *cend,
-                       const DevBox *box, const float *fconst, float4 *fpair);
+                       const DevRep *reps, const float *fconst, float4 *fpair);
 void launch_bonded_kick(hipStream_t st, const Params &P, const Bead *beads, const int2 *chain,
-                        const double4 *cconst, const float4 *fpair, float4 *vel, float4 *acc, const DevBox *box,
-                        long long step, int mode, float4 *fchain_out);
+                        const double4 *cconst, const float4 *fpair, float4 *vel, float4 *acc, const DevRep *reps,
+                        const DevGlobal *glob, int mode, float4 *fchain_out);
 enum : int { KICK_STEP = 0, KICK_INIT = 1, KICK_NONE = 2 };
 void launch_second(hipStream_t st, int n, const Bead *beads, const float4 *acc, float4 *vel, float dt);
 
-// FP64 gathers over the 27-cell stencil; results per block in partial[], reduced by
-// launch_reduce into out[0].
+// FP64 gathers over the 27-cell stencil.  GATHER_U / GATHER_DU write one FP64 value per
+// sorted slot into energy[] (the slot's share of U, or of U(old) - U(scaled) with each
+// replica's trial scale); GATHER_COUNT writes neighbour counts in molecule order;
+// GATHER_LIST appends pairs.
 enum : int { GATHER_U = 0, GATHER_DU = 1, GATHER_COUNT = 2, GATHER_LIST = 3 };
-int gather_blocks(int n);                 // 256-thread blocks over n items
 int cells_blocks(int n, int chunk);       // workgroups of the cell traversals (sm_pair.hip)
 void launch_pair_gather(hipStream_t st, const Params &P, int chunk, int mode, const Bead *sbeads,
                         const uint32_t *skey, const uint32_t *sidx, const int *cstart, const int *cend,
-                        const DevBox *box, const double *uconst, double *partial, int *counts, int *pairs,
+                        const DevRep *reps, const double *uconst, double *energy, int *counts, int *pairs,
                         unsigned long long *npairs, long long cap);
-int chain_blocks(int nchains);
-void launch_chain_energy(hipStream_t st, int nchains, const int2 *chains, const double4 *cconst,
-                         const Bead *beads, const DevBox *box, int delta, double *partial);
-void launch_kinetic(hipStream_t st, int n, const float4 *vel, double *partial);
-// out[k] = sum(partial[k*stride .. k*stride+count_k)) for k < nsum, deterministic order.
-void launch_reduce(hipStream_t st, const double *partial, int stride, const int *counts_host_const,
-                   int nsum, double *out);
-void launch_mc_propose(hipStream_t st, const Params &P, DevBox *box, long long step);
-void launch_mc_decide(hipStream_t st, const Params &P, DevBox *box, const double *partial_pair, int npair,
-                      const double *partial_chain, int nchain);
-void launch_rescale(hipStream_t st, int n, Bead *beads, const DevBox *box);
+// One FP64 value per chain: its energy, or U(old) - U(scaled) when delta != 0.
+void launch_chain_energy(hipStream_t st, int nchains, const int2 *chains, const double4 *cconst, const Bead *beads,
+                         const DevRep *reps, int delta, double *e);
+// One FP64 kinetic energy per bead.
+void launch_kinetic(hipStream_t st, int n, const float4 *vel, double *e);
+void launch_mc_propose(hipStream_t st, int nrep, DevRep *reps, const DevGlobal *glob);
+// du[r*2 + 0] = dU_pair, du[r*2 + 1] = dU_chain of replica r
+void launch_mc_decide(hipStream_t st, const Params &P, DevRep *reps, DevGlobal *glob, const double *du);
+void launch_rescale(hipStream_t st, int n, Bead *beads, const DevRep *reps);
+// fill kernel (graph-capturable replacement for hipMemsetAsync)
+void launch_fill_u32(hipStream_t st, uint32_t *p, size_t count, uint32_t value);
+void launch_set_step(hipStream_t st, DevGlobal *glob, long long step);
 
 // ---- radix sort (sm_sort.hip) ---------------------------------------------------------
 struct SortBuffers {

@@ -1,7 +1,9 @@
-// gfx950 kernels of the MD step: integrate + cell key, reorder + dense cell table, the
-// LDS-staged Laradji-Revalee pair force, CHAIN bonds/bends fused with the Langevin force
-// and the second half-kick, FP64 energy / dU / neighbour gathers, deterministic FP64
-// reductions, and the device-side Monte-Carlo box move.
+// gfx950 kernels of the MD step apart from the cell traversals (sm_pair.hip) and the sort
+// (sm_sort.hip): integrate + cell key, reorder + dense cell table, CHAIN bonds/bends fused
+// with the Langevin force and the second half-kick, FP64 chain energy and kinetic terms,
+// deterministic segmented FP64 sums, and the device-side Monte-Carlo box move.  Every
+// kernel serves all replicas of a context in one launch; a bead finds its replica's box,
+// thermostat and MC state through Bead::rep.
 #include "sm_internal.hpp"
 
 namespace smdev {
@@ -12,7 +14,8 @@ constexpr int kBlock = 256;
 
 // ---------------------------------------------------------------------------------------
 // counter RNG: Philox4x32-10 (Salmon et al., SC'11); counter = (particle, step lo, step hi,
-// tag), key = (seed lo, seed hi).
+// tag), key = (seed lo, seed hi).  The particle word is the bead's index inside its
+// replica, so a replica draws the same numbers alone or batched.
 // ---------------------------------------------------------------------------------------
 __device__ inline uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1)
 {
@@ -53,14 +56,17 @@ __device__ inline bool in_box(double p, double L) { return p >= 0.0 && p <= L; }
 // Verlet::first (verlet.h:289-356) + cell key (cellOpt.h:530-556)
 // ---------------------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_first(int n, Bead *__restrict__ beads, float4 *__restrict__ vel,
-                                                  const float4 *__restrict__ acc, DevBox *__restrict__ box,
-                                                  uint32_t *__restrict__ key, uint32_t *__restrict__ idx,
-                                                  float halfdt, double dt, int integrate)
+                                                  const float4 *__restrict__ acc, const DevRep *__restrict__ reps,
+                                                  DevGlobal *__restrict__ glob, uint32_t *__restrict__ key,
+                                                  uint32_t *__restrict__ idx, float halfdt, double dt, int integrate,
+                                                  long long step_set)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i == 0) glob->step = step_set >= 0 ? step_set : glob->step + 1;   // no other reader in this launch
     if (i >= n) return;
-    const double L0 = box->L[0], L1 = box->L[1], L2 = box->L[2];
     Bead b = beads[i];
+    const DevRep &R = reps[b.rep];
+    const double L0 = R.L[0], L1 = R.L[1], L2 = R.L[2];
     if (integrate && b.type != 0) {
         float4 v = vel[i];
         const float4 a = acc[i];
@@ -79,12 +85,12 @@ __global__ __launch_bounds__(kBlock) void k_first(int n, Bead *__restrict__ bead
         vel[i] = v;
         beads[i] = b;
     }
-    const int nx = box->nc[0], ny = box->nc[1], nz = box->nc[2];
-    if (!(in_box(b.x, L0) && in_box(b.y, L1) && in_box(b.z, L2))) atomicOr(&box->flags, FLAG_BAD_POSITION);
-    const int cx = cell_axis(b.x, box->cs[0], nx);
-    const int cy = cell_axis(b.y, box->cs[1], ny);
-    const int cz = cell_axis(b.z, box->cs[2], nz);
-    key[i] = (uint32_t)(cx + cy * nx + cz * nx * ny);
+    const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
+    if (!(in_box(b.x, L0) && in_box(b.y, L1) && in_box(b.z, L2))) atomicOr(&glob->flags, FLAG_BAD_POSITION);
+    const int cx = cell_axis(b.x, R.cs[0], nx);
+    const int cy = cell_axis(b.y, R.cs[1], ny);
+    const int cz = cell_axis(b.z, R.cs[2], nz);
+    key[i] = (uint32_t)(R.cellbase + (long long)(cx + cy * nx + cz * nx * ny));
     idx[i] = (uint32_t)i;
 }
 
@@ -153,21 +159,21 @@ __global__ __launch_bounds__(kBlock) void k_bonded_kick(int n, const Bead *__res
                                                         const int2 *__restrict__ chain,
                                                         const double4 *__restrict__ cconst,
                                                         const float4 *__restrict__ fpair, float4 *__restrict__ vel,
-                                                        float4 *__restrict__ acc, const DevBox *__restrict__ box,
-                                                        unsigned long long seed, long long step, float gamma,
-                                                        float sigma, float halfdt, int mode,
+                                                        float4 *__restrict__ acc, const DevRep *__restrict__ reps,
+                                                        const DevGlobal *__restrict__ glob, float halfdt, int mode,
                                                         float4 *__restrict__ fchain_out)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const double L0 = box->L[0], L1 = box->L[1], L2 = box->L[2];
+    const Bead p0 = beads[i];
+    const DevRep &R = reps[p0.rep];
+    const double L0 = R.L[0], L1 = R.L[1], L2 = R.L[2];
     double3 fc = make_double3(0.0, 0.0, 0.0);
     const int2 ci = chain[i];
     if (ci.x >= 0) {
         const int start = ci.x, len = ci.y & 0xFFFF, cid = ci.y >> 16;
         const double4 K = cconst[cid];
         const int j = i - start;
-        const Bead p0 = beads[i];
         Bead pm1, pm2, pp1, pp2;
         if (j >= 1) pm1 = beads[i - 1];
         if (j >= 2) pm2 = beads[i - 2];
@@ -198,9 +204,11 @@ __global__ __launch_bounds__(kBlock) void k_bonded_kick(int n, const Bead *__res
         fchain_out[i] = make_float4((float)fc.x, (float)fc.y, (float)fc.z, 0.f);
         return;
     }
-    const uint4 o = philox((uint32_t)i, (uint32_t)step, (uint32_t)((unsigned long long)step >> 32),
-                           mode == KICK_INIT ? TAG_LANGEVIN_INIT : TAG_LANGEVIN, (uint32_t)seed,
-                           (uint32_t)(seed >> 32));
+    const long long step = glob->step;
+    const uint4 o = philox((uint32_t)(i - R.first), (uint32_t)step, (uint32_t)((unsigned long long)step >> 32),
+                           mode == KICK_INIT ? TAG_LANGEVIN_INIT : TAG_LANGEVIN, (uint32_t)R.seed,
+                           (uint32_t)(R.seed >> 32));
+    const float gamma = (float)R.gamma, sigma = (float)R.sigma;
     float4 v = vel[i];
     const float4 fp = fpair[i];
     float4 a;
@@ -211,7 +219,7 @@ __global__ __launch_bounds__(kBlock) void k_bonded_kick(int n, const Bead *__res
     a.x += (float)fc.x; a.y += (float)fc.y; a.z += (float)fc.z;
     a.w = 0.f;
     acc[i] = a;
-    if (mode == KICK_STEP && beads[i].type != 0) {
+    if (mode == KICK_STEP && p0.type != 0) {
         v.x += a.x * halfdt;
         v.y += a.y * halfdt;
         v.z += a.z * halfdt;
@@ -235,7 +243,8 @@ __global__ __launch_bounds__(kBlock) void k_second(int n, const Bead *__restrict
 }
 
 // ---------------------------------------------------------------------------------------
-// FP64 reductions: fixed-shape trees, so the result is independent of scheduling.
+// Segmented FP64 sums with fixed-shape trees: the result is independent of scheduling
+// and of the other replicas.
 // ---------------------------------------------------------------------------------------
 __device__ inline double block_sum(double v, double *lds)
 {
@@ -250,19 +259,27 @@ __device__ inline double block_sum(double v, double *lds)
     return r;
 }
 
-__device__ inline double reduce_partials(const double *p, int count, double *lds)
-{
-    double s = 0.0;
-    for (int i = threadIdx.x; i < count; i += kBlock) s += p[i];
-    return block_sum(s, lds);
-}
-
-__global__ __launch_bounds__(kBlock) void k_reduce(const double *__restrict__ partial, int count,
-                                                   double *__restrict__ out)
+__global__ __launch_bounds__(kBlock) void k_seg_tiles(const double *__restrict__ x, const int2 *__restrict__ tile,
+                                                      double *__restrict__ tilesum)
 {
     __shared__ double lds[kBlock];
-    const double s = reduce_partials(partial, count, lds);
-    if (threadIdx.x == 0) *out = s;
+    const int2 t = tile[blockIdx.x];
+    double s = 0.0;
+    for (int i = t.x + (int)threadIdx.x; i < t.y; i += kBlock) s += x[i];
+    const double b = block_sum(s, lds);
+    if (threadIdx.x == 0) tilesum[blockIdx.x] = b;
+}
+
+__global__ __launch_bounds__(kBlock) void k_seg_final(const double *__restrict__ tilesum,
+                                                      const int *__restrict__ rep_first, double *__restrict__ out,
+                                                      int stride)
+{
+    __shared__ double lds[kBlock];
+    const int r = blockIdx.x;
+    double s = 0.0;
+    for (int t = rep_first[r] + (int)threadIdx.x; t < rep_first[r + 1]; t += kBlock) s += tilesum[t];
+    const double b = block_sum(s, lds);
+    if (threadIdx.x == 0) out[(size_t)r * stride] = b;
 }
 
 // CHAIN energies in FP64 (harmonicP MD.h:422-431, bendP MD.h:769-789, walk
@@ -295,133 +312,143 @@ __device__ inline double bend_p(const double a[3], const double b[3], double c0,
 __global__ __launch_bounds__(kBlock) void k_chain_energy(int nch, const int2 *__restrict__ chains,
                                                          const double4 *__restrict__ cconst,
                                                          const Bead *__restrict__ beads,
-                                                         const DevBox *__restrict__ box, int delta,
-                                                         double *__restrict__ partial)
+                                                         const DevRep *__restrict__ reps, int delta,
+                                                         double *__restrict__ e)
 {
-    __shared__ double lds[kBlock];
     const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nch) return;
+    const int2 ch = chains[c];
+    const int start = ch.x, len = ch.y & 0xFFFF;
+    const DevRep &R = reps[beads[start].rep];
+    const double L[3] = {R.L[0], R.L[1], R.L[2]};
+    const double S[3] = {R.scale[0], R.scale[1], R.scale[2]};
+    const double4 K = cconst[ch.y >> 16];
     double accu = 0.0;
-    if (c < nch) {
-        const double L[3] = {box->L[0], box->L[1], box->L[2]};
-        const double S[3] = {box->scale[0], box->scale[1], box->scale[2]};
-        const int2 ch = chains[c];
-        const int start = ch.x, len = ch.y & 0xFFFF;
-        const double4 K = cconst[ch.y >> 16];
-        for (int l = start; l <= start + len - 3; l++) {
-            const bool last = (l == start + len - 3);
-            double da[3], db[3];
-            mi3(beads[l], beads[l + 1], L, da);
-            mi3(beads[l + 1], beads[l + 2], L, db);
-            double u = harm_p(da, K.x, K.y);
-            if (last) u += harm_p(db, K.x, K.y);
-            u += bend_p(da, db, K.z, K.w);
-            if (delta) {
-                for (int d = 0; d < 3; d++) { da[d] *= S[d]; db[d] *= S[d]; }
-                double un = harm_p(da, K.x, K.y);
-                if (last) un += harm_p(db, K.x, K.y);
-                un += bend_p(da, db, K.z, K.w);
-                accu += (u - un);
-            } else {
-                accu += u;
-            }
+    for (int l = start; l <= start + len - 3; l++) {
+        const bool last = (l == start + len - 3);
+        double da[3], db[3];
+        mi3(beads[l], beads[l + 1], L, da);
+        mi3(beads[l + 1], beads[l + 2], L, db);
+        double u = harm_p(da, K.x, K.y);
+        if (last) u += harm_p(db, K.x, K.y);
+        u += bend_p(da, db, K.z, K.w);
+        if (delta) {
+            for (int d = 0; d < 3; d++) { da[d] *= S[d]; db[d] *= S[d]; }
+            double un = harm_p(da, K.x, K.y);
+            if (last) un += harm_p(db, K.x, K.y);
+            un += bend_p(da, db, K.z, K.w);
+            accu += (u - un);
+        } else {
+            accu += u;
         }
     }
-    const double b = block_sum(accu, lds);
-    if (threadIdx.x == 0) partial[blockIdx.x] = b;
+    e[c] = accu;
 }
 
-// Kinetic::compute (dataCollection.h:667-673)
-__global__ __launch_bounds__(kBlock) void k_kinetic(int n, const float4 *__restrict__ vel,
-                                                    double *__restrict__ partial)
+// Kinetic::compute (dataCollection.h:667-673), one term per bead
+__global__ __launch_bounds__(kBlock) void k_kinetic(int n, const float4 *__restrict__ vel, double *__restrict__ e)
 {
-    __shared__ double lds[kBlock];
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    double e = 0.0;
-    if (i < n) {
-        const float4 v = vel[i];
-        const double x = v.x, y = v.y, z = v.z;
-        e = (x * x + y * y + z * z) / 2.0;
+    if (i >= n) return;
+    const float4 v = vel[i];
+    const double x = v.x, y = v.y, z = v.z;
+    e[i] = (x * x + y * y + z * z) / 2.0;
+}
+
+// ---------------------------------------------------------------------------------------
+// Monte-Carlo constant-tension move (MD.cpp:589-721), decided on the device, one thread
+// per replica.  A replica with delta_lxy == 0 makes no trial.
+// ---------------------------------------------------------------------------------------
+__global__ void k_mc_propose(int nrep, DevRep *reps, const DevGlobal *__restrict__ glob)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrep) return;
+    DevRep &R = reps[r];
+    R.accept = 0;
+    if (R.delta_lxy == 0) {
+        R.scale[0] = R.scale[1] = R.scale[2] = 1.0;
+        return;
     }
-    const double b = block_sum(e, lds);
-    if (threadIdx.x == 0) partial[blockIdx.x] = b;
-}
-
-// ---------------------------------------------------------------------------------------
-// Monte-Carlo constant-tension move (MD.cpp:589-721), decided on the device.
-// ---------------------------------------------------------------------------------------
-__global__ void k_mc_propose(DevBox *box, unsigned long long seed, long long step, double dL)
-{
+    const long long step = glob->step;
     const uint4 o = philox(0u, (uint32_t)step, (uint32_t)((unsigned long long)step >> 32), TAG_MC,
-                           (uint32_t)seed, (uint32_t)(seed >> 32));
+                           (uint32_t)R.seed, (uint32_t)(R.seed >> 32));
     const double psi = psi64(o.x);
-    box->u_accept = ((o.y >> 5) * 67108864.0 + (o.z >> 6)) * (1.0 / 9007199254740992.0);
-    const double Lx = box->L[0], Ly = box->L[1], Lz = box->L[2];
-    const double fx = dL * psi, fy = fx;
+    R.u_accept = ((o.y >> 5) * 67108864.0 + (o.z >> 6)) * (1.0 / 9007199254740992.0);
+    const double Lx = R.L[0], Ly = R.L[1], Lz = R.L[2];
+    const double fx = R.delta_lxy * psi, fy = fx;
     const double fz = (Lx * Ly) / ((Lx + fx) * (Ly + fy));
-    box->fluct = fx;
-    box->newL[0] = Lx + fx;
-    box->newL[1] = Ly + fy;
-    box->newL[2] = Lz * fz;
-    box->scale[0] = box->newL[0] / Lx;
-    box->scale[1] = box->newL[1] / Ly;
-    box->scale[2] = box->newL[2] / Lz;
-    box->accept = 0;
+    R.fluct = fx;
+    R.newL[0] = Lx + fx;
+    R.newL[1] = Ly + fy;
+    R.newL[2] = Lz * fz;
+    R.scale[0] = R.newL[0] / Lx;
+    R.scale[1] = R.newL[1] / Ly;
+    R.scale[2] = R.newL[2] / Lz;
 }
 
-__global__ __launch_bounds__(kBlock) void k_mc_decide(DevBox *box, const double *__restrict__ pp, int np,
-                                                      const double *__restrict__ pc, int nc, double T,
-                                                      double tension, double dL, double cutoff)
+__global__ void k_mc_decide(int nrep, DevRep *reps, DevGlobal *glob, const double *__restrict__ du, double cutoff)
 {
-    __shared__ double lds[kBlock];
-    const double dup = reduce_partials(pp, np, lds);
-    const double duc = reduce_partials(pc, nc, lds);
-    if (threadIdx.x != 0) return;
-    const double Lx = box->L[0], Ly = box->L[1];
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrep) return;
+    DevRep &R = reps[r];
+    if (R.delta_lxy == 0) return;
+    const double dup = du[2 * r], duc = du[2 * r + 1];
+    const double dL = R.delta_lxy;
+    const double Lx = R.L[0], Ly = R.L[1];
     double dP = dup;
     dP += duc;
-    if (tension != 0) dP += tension * ((box->newL[0] * box->newL[1]) - (Lx * Ly));
-    double D = dP / T;
+    if (R.tension != 0) dP += R.tension * ((R.newL[0] * R.newL[1]) - (Lx * Ly));
+    double D = dP / R.temperature;
     D = exp(D);
-    bool acc = (D >= box->u_accept) || (-dP <= 0);
+    bool acc = (D >= R.u_accept) || (-dP <= 0);
     int nc3[3];
     long long cells = 1;
     for (int d = 0; d < 3; d++) {
-        nc3[d] = (int)(box->newL[d] / cutoff);
+        nc3[d] = (int)(R.newL[d] / cutoff);
         cells *= nc3[d];
     }
-    if (acc && (cells > box->capacity || nc3[0] < 3 || nc3[1] < 3 || nc3[2] < 3)) {
-        box->flags |= FLAG_CELL_OVERFLOW;
+    if (acc && (cells > R.capacity || nc3[0] < 3 || nc3[1] < 3 || nc3[2] < 3)) {
+        atomicOr(&glob->flags, FLAG_CELL_OVERFLOW);
         acc = false;
     }
-    box->dU_pair = dup;
-    box->dU_chain = duc;
+    R.dU_pair = dup;
+    R.dU_chain = duc;
     const double interval = dL / 101.0;
-    const long long bin = (long long)((box->fluct + dL) / interval);
+    const long long bin = (long long)((R.fluct + dL) / interval);
     if (acc) {
         for (int d = 0; d < 3; d++) {
-            box->L[d] = box->newL[d];
-            box->nc[d] = nc3[d];
-            box->cs[d] = box->newL[d] / nc3[d];
+            R.L[d] = R.newL[d];
+            R.nc[d] = nc3[d];
+            R.cs[d] = R.newL[d] / nc3[d];
         }
-        box->accepted++;
-        if (bin >= 0 && bin < 256) box->hist_accept[bin] += 1.0;
-    } else if (bin >= 0 && bin < 256) {
-        box->hist_reject[bin] += 1.0;
+        R.accepted++;
+        if (bin >= 0 && bin < kHistBins) R.hist_accept[bin] += 1.0;
+    } else if (bin >= 0 && bin < kHistBins) {
+        R.hist_reject[bin] += 1.0;
     }
-    box->accept = acc ? 1 : 0;
-    box->trials++;
+    R.accept = acc ? 1 : 0;
+    R.trials++;
 }
 
-__global__ __launch_bounds__(kBlock) void k_rescale(int n, Bead *__restrict__ beads, const DevBox *__restrict__ box)
+__global__ __launch_bounds__(kBlock) void k_rescale(int n, Bead *__restrict__ beads, const DevRep *__restrict__ reps)
 {
-    if (!box->accept) return;
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     Bead b = beads[i];
-    b.x *= box->scale[0];
-    b.y *= box->scale[1];
-    b.z *= box->scale[2];
+    const DevRep &R = reps[b.rep];
+    if (!R.accept) return;
+    b.x *= R.scale[0];
+    b.y *= R.scale[1];
+    b.z *= R.scale[2];
     beads[i] = b;
+}
+
+__global__ void k_set_step(DevGlobal *glob, long long step) { glob->step = step; }
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t *__restrict__ p, size_t count, uint32_t value)
+{
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < count) p[i] = value;
 }
 
 inline int blocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
@@ -431,11 +458,11 @@ inline int blocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 // ---------------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------------
-void launch_first(hipStream_t st, const Params &P, Bead *beads, float4 *vel, const float4 *acc, const DevBox *box,
-                  uint32_t *key, uint32_t *idx, int integrate)
+void launch_first(hipStream_t st, const Params &P, Bead *beads, float4 *vel, const float4 *acc, const DevRep *reps,
+                  DevGlobal *glob, uint32_t *key, uint32_t *idx, int integrate, long long step_set)
 {
-    hipLaunchKernelGGL(k_first, dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, beads, vel, acc,
-                       const_cast<DevBox *>(box), key, idx, (float)(0.5 * P.dt), P.dt, integrate);
+    hipLaunchKernelGGL(k_first, dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, beads, vel, acc, reps, glob, key, idx,
+                       (float)(0.5 * P.dt), P.dt, integrate, step_set);
 }
 
 void launch_reorder(hipStream_t st, int n, const uint32_t *skey, const uint32_t *sidx, const Bead *beads,
@@ -445,12 +472,11 @@ void launch_reorder(hipStream_t st, int n, const uint32_t *skey, const uint32_t 
 }
 
 void launch_bonded_kick(hipStream_t st, const Params &P, const Bead *beads, const int2 *chain, const double4 *cconst,
-                        const float4 *fpair, float4 *vel, float4 *acc, const DevBox *box, long long step, int mode,
-                        float4 *fchain_out)
+                        const float4 *fpair, float4 *vel, float4 *acc, const DevRep *reps, const DevGlobal *glob,
+                        int mode, float4 *fchain_out)
 {
-    const double sigma = sqrt((6.0 * P.temperature * P.gamma) / P.dt);
     hipLaunchKernelGGL(k_bonded_kick, dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, beads, chain, cconst, fpair, vel,
-                       acc, box, P.seed, step, (float)P.gamma, (float)sigma, (float)(P.dt * 0.5), mode, fchain_out);
+                       acc, reps, glob, (float)(P.dt * 0.5), mode, fchain_out);
 }
 
 void launch_second(hipStream_t st, int n, const Bead *beads, const float4 *acc, float4 *vel, float halfdt)
@@ -458,42 +484,51 @@ void launch_second(hipStream_t st, int n, const Bead *beads, const float4 *acc, 
     hipLaunchKernelGGL(k_second, dim3(blocks(n)), dim3(kBlock), 0, st, n, beads, acc, vel, halfdt);
 }
 
-int gather_blocks(int n) { return blocks(n > 0 ? n : 1); }
-
-int chain_blocks(int nchains) { return blocks(nchains > 0 ? nchains : 1); }
+void launch_seg_sum(hipStream_t st, const SegTiles &S, const double *x, double *out, int stride)
+{
+    if (S.ntiles > 0)
+        hipLaunchKernelGGL(k_seg_tiles, dim3(S.ntiles), dim3(kBlock), 0, st, x, S.tile, S.tilesum);
+    hipLaunchKernelGGL(k_seg_final, dim3(S.nrep), dim3(kBlock), 0, st, S.tilesum, S.rep_first, out, stride);
+}
 
 void launch_chain_energy(hipStream_t st, int nchains, const int2 *chains, const double4 *cconst, const Bead *beads,
-                         const DevBox *box, int delta, double *partial)
+                         const DevRep *reps, int delta, double *e)
 {
-    hipLaunchKernelGGL(k_chain_energy, dim3(chain_blocks(nchains)), dim3(kBlock), 0, st, nchains, chains, cconst,
-                       beads, box, delta, partial);
+    if (nchains <= 0) return;
+    hipLaunchKernelGGL(k_chain_energy, dim3(blocks(nchains)), dim3(kBlock), 0, st, nchains, chains, cconst, beads,
+                       reps, delta, e);
 }
 
-void launch_kinetic(hipStream_t st, int n, const float4 *vel, double *partial)
+void launch_kinetic(hipStream_t st, int n, const float4 *vel, double *e)
 {
-    hipLaunchKernelGGL(k_kinetic, dim3(gather_blocks(n)), dim3(kBlock), 0, st, n, vel, partial);
+    hipLaunchKernelGGL(k_kinetic, dim3(blocks(n)), dim3(kBlock), 0, st, n, vel, e);
 }
 
-void launch_reduce_sum(hipStream_t st, const double *partial, int count, double *out)
+void launch_mc_propose(hipStream_t st, int nrep, DevRep *reps, const DevGlobal *glob)
 {
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, st, partial, count, out);
+    hipLaunchKernelGGL(k_mc_propose, dim3(blocks(nrep)), dim3(kBlock), 0, st, nrep, reps, glob);
 }
 
-void launch_mc_propose(hipStream_t st, const Params &P, DevBox *box, long long step)
+void launch_mc_decide(hipStream_t st, const Params &P, DevRep *reps, DevGlobal *glob, const double *du)
 {
-    hipLaunchKernelGGL(k_mc_propose, dim3(1), dim3(1), 0, st, box, P.seed, step, P.delta_lxy);
+    hipLaunchKernelGGL(k_mc_decide, dim3(blocks(P.nrep)), dim3(kBlock), 0, st, P.nrep, reps, glob, du, P.cutoff);
 }
 
-void launch_mc_decide(hipStream_t st, const Params &P, DevBox *box, const double *pp, int np, const double *pc,
-                      int nc)
+void launch_rescale(hipStream_t st, int n, Bead *beads, const DevRep *reps)
 {
-    hipLaunchKernelGGL(k_mc_decide, dim3(1), dim3(kBlock), 0, st, box, pp, np, pc, nc, P.temperature, P.tension,
-                       P.delta_lxy, P.cutoff);
+    hipLaunchKernelGGL(k_rescale, dim3(blocks(n)), dim3(kBlock), 0, st, n, beads, reps);
 }
 
-void launch_rescale(hipStream_t st, int n, Bead *beads, const DevBox *box)
+void launch_set_step(hipStream_t st, DevGlobal *glob, long long step)
 {
-    hipLaunchKernelGGL(k_rescale, dim3(blocks(n)), dim3(kBlock), 0, st, n, beads, box);
+    hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, st, glob, step);
+}
+
+void launch_fill_u32(hipStream_t st, uint32_t *p, size_t count, uint32_t value)
+{
+    if (count == 0) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((count + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, p, count,
+                       value);
 }
 
 }  // namespace smdev

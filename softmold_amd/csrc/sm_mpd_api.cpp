@@ -1,0 +1,227 @@
+// C ABI of the host-side `.mpd` system files and the two reference generators: the
+// part of include/softmold_hip.h that needs no GPU.
+#include "../../include/softmold_hip.h"
+#include "mpd_io.hpp"
+#include "sm_errors.hpp"
+
+#include <cstring>
+#include <functional>
+#include <string>
+#include <vector>
+
+using namespace smapi;
+
+struct sm_mpd {
+    smio::System sys;
+    std::vector<int32_t> chain;
+    std::vector<double> chain_const;
+};
+
+namespace {
+
+const struct { const char *name; smio::Cmd cmd; } kScalars[] = {
+    {"gamma", smio::GAMMA}, {"initialTemp", smio::INITIALTEMP}, {"finalTemp", smio::FINALTEMP},
+    {"seed", smio::SEED}, {"cutoff", smio::CUTOFF}, {"initialTime", smio::INITIALTIME},
+    {"finalTime", smio::FINALTIME}, {"deltaT", smio::DELTAT}, {"storeInterval", smio::STOREINTERVAL},
+    {"measureInterval", smio::MEASUREINTERVAL}, {"deltaLXY", smio::DELTALXY},
+    {"removeSolvent", smio::REMOVESOLVENT}, {"tempStepInterval", smio::TEMPSTEPINTERVAL},
+    {"tension", smio::TENSION}, {"nTypes", smio::NTYPES}, {"nParticles", smio::NPARTICLES},
+    {"nMolecules", smio::NMOLECULES}};
+
+double *scalar_ref(smio::System &s, smio::Cmd c)
+{
+    switch (c) {
+    case smio::GAMMA: return &s.gamma;
+    case smio::INITIALTEMP: return &s.initialTemp;
+    case smio::FINALTEMP: return &s.finalTemp;
+    case smio::CUTOFF: return &s.cutoff;
+    case smio::INITIALTIME: return &s.initialTime;
+    case smio::FINALTIME: return &s.finalTime;
+    case smio::DELTAT: return &s.deltaT;
+    case smio::STOREINTERVAL: return &s.storeInterval;
+    case smio::MEASUREINTERVAL: return &s.measureInterval;
+    case smio::DELTALXY: return &s.deltaLXY;
+    case smio::REMOVESOLVENT: return &s.removeSolvent;
+    case smio::TEMPSTEPINTERVAL: return &s.tempStepInterval;
+    case smio::TENSION: return &s.tension;
+    default: return nullptr;
+    }
+}
+
+int io_guard(const std::function<void()> &f)
+{
+    try {
+        f();
+        return SM_OK;
+    } catch (const ApiError &e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return SM_ERR_IO;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int sm_mpd_read(const char *name, sm_mpd **out)
+{
+    if (!name || !out) {
+        g_err = "null argument";
+        return SM_ERR_INVALID;
+    }
+    *out = nullptr;
+    return io_guard([&] {
+        auto *m = new sm_mpd();
+        try {
+            m->sys = smio::read_file(std::string(name) + ".mpd");
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *out = m;
+    });
+}
+
+int sm_mpd_write(const sm_mpd *m, const char *name)
+{
+    if (!m || !name) {
+        g_err = "null argument";
+        return SM_ERR_INVALID;
+    }
+    return io_guard([&] { smio::write_file(m->sys, std::string(name) + ".mpd"); });
+}
+
+int sm_mpd_free(sm_mpd *m)
+{
+    delete m;
+    return SM_OK;
+}
+
+int sm_mpd_system(const sm_mpd *mc, sm_system *out)
+{
+    if (!mc || !out) {
+        g_err = "null argument";
+        return SM_ERR_INVALID;
+    }
+    sm_mpd *m = const_cast<sm_mpd *>(mc);
+    return guarded([&] {
+        const smio::System &s = m->sys;
+        m->chain.clear();
+        m->chain_const.clear();
+        for (const auto &mo : s.mol) {
+            if (mo.type != smio::CHAIN)
+                unsupported("molecule type " + std::to_string(mo.type) + " is not implemented (CHAIN only)");
+            for (const auto &b : mo.bonds) {
+                m->chain.insert(m->chain.end(), {b[0], b[1], b[2]});
+                m->chain_const.insert(m->chain_const.end(), mo.constants.begin(), mo.constants.end());
+            }
+        }
+        if (s.gamma <= 0) unsupported("gamma must be > 0 (per-type gammaType is not implemented)");
+        std::memset(out, 0, sizeof *out);
+        out->n_particles = s.nParticles;
+        out->n_types = s.nTypes;
+        out->pos = s.pos.data();
+        out->type = s.type.data();
+        out->vel = s.vel.data();
+        out->acc = nullptr;
+        out->n_chain_blocks = (int32_t)(m->chain.size() / 3);
+        out->chain = m->chain.data();
+        out->chain_const = m->chain_const.data();
+        out->fconst = s.fconst.data();
+        out->uconst = s.uconst.data();
+        for (int d = 0; d < 3; d++) out->box[d] = s.size[d];
+        out->cutoff = s.cutoff;
+        out->dt = s.deltaT;
+        out->gamma = s.gamma;
+        out->temperature = s.initialTemp;
+        out->delta_lxy = s.deltaLXY;
+        out->tension = s.tension;
+        out->seed = (uint64_t)(uint32_t)s.seed;
+    });
+}
+
+int sm_mpd_get(const sm_mpd *m, const char *key, double *value)
+{
+    if (!m || !key || !value) {
+        g_err = "null argument";
+        return SM_ERR_INVALID;
+    }
+    for (const auto &k : kScalars) {
+        if (std::strcmp(k.name, key) != 0) continue;
+        smio::System &s = const_cast<sm_mpd *>(m)->sys;
+        switch (k.cmd) {
+        case smio::SEED: *value = s.seed; return SM_OK;
+        case smio::NTYPES: *value = s.nTypes; return SM_OK;
+        case smio::NPARTICLES: *value = s.nParticles; return SM_OK;
+        case smio::NMOLECULES: *value = s.nMolecules; return SM_OK;
+        default: *value = *scalar_ref(s, k.cmd); return SM_OK;
+        }
+    }
+    g_err = std::string("unknown scalar key ") + key;
+    return SM_ERR_INVALID;
+}
+
+int sm_mpd_set(sm_mpd *m, const char *key, double value)
+{
+    if (!m || !key) {
+        g_err = "null argument";
+        return SM_ERR_INVALID;
+    }
+    for (const auto &k : kScalars) {
+        if (std::strcmp(k.name, key) != 0) continue;
+        double *p = scalar_ref(m->sys, k.cmd);
+        if (k.cmd == smio::SEED) m->sys.seed = (int)value;
+        else if (!p) {
+            g_err = std::string(key) + " is not settable";
+            return SM_ERR_INVALID;
+        } else *p = value;
+        m->sys.present[k.cmd] = true;
+        return SM_OK;
+    }
+    g_err = std::string("unknown scalar key ") + key;
+    return SM_ERR_INVALID;
+}
+
+int sm_mpd_set_state(sm_mpd *m, const double *pos, const double *vel, const double box[3])
+{
+    if (!m) {
+        g_err = "null argument";
+        return SM_ERR_INVALID;
+    }
+    const size_t n3 = 3 * (size_t)m->sys.nParticles;
+    if (pos) std::memcpy(m->sys.pos.data(), pos, n3 * sizeof(double));
+    if (vel) std::memcpy(m->sys.vel.data(), vel, n3 * sizeof(double));
+    if (box) for (int d = 0; d < 3; d++) m->sys.size[d] = box[d];
+    return SM_OK;
+}
+
+int sm_gen_liposome(int32_t seed, int32_t n_lipids, double rho, sm_mpd **out)
+{
+    if (!out || n_lipids <= 0 || !(rho > 0)) {
+        g_err = "bad argument";
+        return SM_ERR_INVALID;
+    }
+    return io_guard([&] {
+        auto *m = new sm_mpd();
+        m->sys = smio::gen_liposome(seed, n_lipids, rho);
+        *out = m;
+    });
+}
+
+int sm_gen_bilayer(int32_t seed, int32_t n_lipids, double rho, sm_mpd **out)
+{
+    if (!out || n_lipids <= 0 || !(rho > 0)) {
+        g_err = "bad argument";
+        return SM_ERR_INVALID;
+    }
+    return io_guard([&] {
+        auto *m = new sm_mpd();
+        m->sys = smio::gen_bilayer(seed, n_lipids, rho);
+        *out = m;
+    });
+}
+
+}  // extern "C"

@@ -15,12 +15,14 @@
 //      are in flight together;
 //   3. tpp = 64/H lanes serve each of the H home particles, each lane walking every
 //      tpp-th candidate; partial results meet in LDS and are combined in lane order.
-// Force: candidates are staged twice -- FP64 imaged positions, and FP32 positions
-// relative to the home cell centre.  A lane first screens kList candidates with the FP32
-// copy against a cutoff widened by more than the FP32 error, keeping the hits in a
-// per-lane LDS list; then, on the hits only, it takes the FP64 difference, applies the
-// exact FP64 test r^2 < rc^2 (no contraction, as MD.h:798-810) and evaluates the force
-// law in FP32 with a Newton-corrected r.  Gathers: FP64 throughout, each pair's
+// Force, one phase (M_FORCE1, the default): candidates are staged as FP64 imaged
+// positions; a lane takes the FP64 difference, applies the exact FP64 test r^2 < rc^2
+// (no contraction, as MD.h:798-810) and under that branch evaluates the force law in FP32
+// with a Newton-corrected r.  Force, two phase (M_FORCE, kept as a tuning variant):
+// candidates are staged a second time as FP32 positions relative to the home cell
+// centre; a lane first screens kList candidates with the FP32 copy against a cutoff
+// widened by more than the FP32 error, keeping the hits in a per-lane LDS list, then runs
+// the exact FP64 test and the force law on the hits only.  Gathers: FP64 throughout, each pair's
 // difference taken in the reference's direction -- the particle of the cell that owns
 // the pair in the 13-cell half shell (cellOpt.h:715) minus the imaged other.
 #include "sm_internal.hpp"
@@ -64,14 +66,27 @@ __device__ inline int xcd_block(int b, int nblocks)
     return x * q + min(x, r) + (b >> 3);
 }
 
+// r^2 rounded as the reference computes it, (x*x + y*y) + z*z with every product and sum
+// rounded separately: HIP contracts a*b + c into one FMA by default, which would decide
+// pairs at the cutoff differently from the reference (MD.h:798-810 on x86-64 has no FMA).
 __device__ inline double sq3(double x, double y, double z)
 {
-    return __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z));
+#pragma clang fp contract(off)
+    return (x * x + y * y) + z * z;
+}
+
+// a*s - b*s with both products rounded, as the reference scales both positions before
+// differencing them (cellOpt.h:1098-1105)
+__device__ inline double scaled_diff(double a, double b, double s)
+{
+#pragma clang fp contract(off)
+    return a * s - b * s;
 }
 
 // Potential<double> (MD.h:895-930)
 __device__ inline double pair_u(double r2, const double *C)
 {
+#pragma clang fp contract(off)
     const double dr = sqrt(r2);
     double pot;
     if (dr <= C[0]) {
@@ -95,9 +110,40 @@ __device__ constexpr uint32_t forward_mask()
     return m;
 }
 
+// One candidate of the force kernels: the exact FP64 test r^2 < rc^2 (MD.h:798-810), then
+// the force law (MD.h:811-848) in FP32.  With r0 = r2f * rsq(r2f) (exact as an FP64
+// product), r = r0 + corr is one Newton step against the FP64 residual r^2 - r0^2, and
+// m = C - r is formed as fma(-r2f, rsq, C) - corr, so it carries no FP32 rounding of r^2
+// or r0 (near r_m the force is C1 m with C1 ~ 400).  1/r is rsq(r2f) refined by one Newton
+// step against r.
+__device__ inline void force_term(const Bead &hb, const Cand &q, bool other, double cut2, const float *Cf, float &fx,
+                                  float &fy, float &fz)
+{
+    const double dxd = hb.x - q.x;
+    const double dyd = hb.y - q.y;
+    const double dzd = hb.z - q.z;
+    const double r2 = sq3(dxd, dyd, dzd);
+    if (r2 < cut2 && other) {
+        const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
+        const float r2f = (float)r2;
+        const float ri0 = __builtin_amdgcn_rsqf(r2f);
+        const double r0d = (double)r2f * (double)ri0;
+        const float corr = (float)fma(-r0d, r0d, r2) * (0.5f * ri0);
+        const float r = (float)r0d + corr;
+        const float ri = ri0 * (2.0f - r * ri0);
+        const float *C = Cf + 6 * q.t;
+        const int o = (r >= C[0]) ? 3 : 0;       // int(r/r_m)*3, clamped
+        const float m = fmaf(-r2f, ri0, C[o]) - corr;
+        const float mag = ((C[o + 1] - C[o + 2] * m) * m) * ri;
+        fx += dx * mag;
+        fy += dy * mag;
+        fz += dz * mag;
+    }
+}
+
 struct Out {
     float4 *force;            // M_FORCE, original order
-    double *partial;          // M_U, M_DU: one FP64 sum per workgroup
+    double *energy;           // M_U, M_DU: one FP64 value per sorted slot
     int *counts;              // M_COUNT, original order
     int *pairs;               // M_LIST
     unsigned long long *npairs;
@@ -108,7 +154,7 @@ template <int MODE>
 __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(4))) void k_cells(
     int n, int nT, int chunk, int nblocks, double cut2, float cut2pre, const Bead *__restrict__ sb,
     const uint32_t *__restrict__ skey, const uint32_t *__restrict__ sidx, const int *__restrict__ cstart,
-    const int *__restrict__ cend, const DevBox *__restrict__ box, const float *__restrict__ fconst,
+    const int *__restrict__ cend, const DevRep *__restrict__ reps, const float *__restrict__ fconst,
     const double *__restrict__ uconst, Out out)
 {
     constexpr bool kF = MODE == M_FORCE || MODE == M_FORCE1;   // force (two-phase / one-phase)
@@ -121,7 +167,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     __shared__ float4 sRel[k2P ? kWaves : 1][k2P ? kMaxC : 1];
     __shared__ uint8_t sList[k2P ? kWaves : 1][k2P ? kList : 1][kWave];   // per-lane hit lists, lane-minor
     __shared__ float redf[kF ? kWaves : 1][3][kWave];
-    __shared__ double redd[kWaves * kWave];
+    __shared__ double redd[kF ? 1 : kWaves][kWave];
     __shared__ int redi[kWaves][kWave];
 
     if (kF) {
@@ -136,14 +182,9 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     const long long a0 = ((long long)xcd_block(blockIdx.x, nblocks) * kWaves + wave) * chunk;
     const int a = (int)min(a0, (long long)n);
     const int aend = min(a + chunk, n);
-    const int nx = box->nc[0], ny = box->nc[1], nz = box->nc[2];
-    const double Lx = box->L[0], Ly = box->L[1], Lz = box->L[2];
-    const double csx = box->cs[0], csy = box->cs[1], csz = box->cs[2];
-    const double ax = box->scale[0], ay = box->scale[1], az = box->scale[2];
     constexpr uint32_t fwd = forward_mask();
     Cand *Cw = sCand[wave];
     float4 *Rw = sRel[k2P ? wave : 0];
-    double accu = 0.0;
 
     for (int base = a; base < aend; base += kWave) {
         const int s = base + lane;
@@ -153,8 +194,16 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
         while (starts) {
             const int s0 = base + __ffsll((long long)starts) - 1;
             starts &= starts - 1;
-            const uint32_t k = skey[s0];
-            const int e0 = cend[k];
+            const uint32_t kg = skey[s0];
+            const int e0 = cend[kg];
+            // the home cell's replica: box, grid and key offset (wave-uniform)
+            const DevRep &R = reps[sb[s0].rep];
+            const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
+            const double Lx = R.L[0], Ly = R.L[1], Lz = R.L[2];
+            const double csx = R.cs[0], csy = R.cs[1], csz = R.cs[2];
+            const double ax = R.scale[0], ay = R.scale[1], az = R.scale[2];
+            const uint32_t kbase = (uint32_t)R.cellbase;
+            const uint32_t k = kg - kbase;
             const int kx = (int)(k % (uint32_t)nx);
             const int ky = (int)((k / (uint32_t)nx) % (uint32_t)ny);
             const int kz = (int)(k / (uint32_t)(nx * ny));
@@ -168,7 +217,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                 X0 = X0 < 0 ? X0 + nx : (X0 >= nx ? X0 - nx : X0);
                 Y0 = Y0 < 0 ? Y0 + ny : (Y0 >= ny ? Y0 - ny : Y0);
                 Z0 = Z0 < 0 ? Z0 + nz : (Z0 >= nz ? Z0 - nz : Z0);
-                const uint32_t kk = (uint32_t)(X0 + Y0 * nx + Z0 * nx * ny);
+                const uint32_t kk = kbase + (uint32_t)(X0 + Y0 * nx + Z0 * nx * ny);
                 const int c0 = cstart[kk];
                 const int c1 = cend[kk];
                 if ((unsigned)c0 < (unsigned)n && skey[c0] == kk) {
@@ -200,6 +249,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                 const uint32_t me = act ? sidx[s0 + h0 + hi] : 0u;
                 const float hfx = (float)(hb.x - ox0), hfy = (float)(hb.y - oy0), hfz = (float)(hb.z - oz0);
                 float fx = 0.f, fy = 0.f, fz = 0.f;
+                double accu = 0.0;
                 int cnt = 0;
                 for (int cb = 0; cb < total; cb += kMaxC) {
                     const int ce = min(total, cb + kMaxC);
@@ -243,30 +293,17 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                     }
                     wave_lds_sync();
                     if (MODE == M_FORCE1 && act) {
-                        // one phase: FP64 test, force law under the branch
+                        // one phase: FP64 test, force law under the branch; two candidates
+                        // per trip so both LDS reads are in flight before either is used
                         const float *Cf = sCf + 6 * hb.type * nT;
-                        for (int j = cb + sub; j < ce; j += tpp) {
-                            const Cand q = Cw[j - cb];
-                            const double dxd = hb.x - q.x;
-                            const double dyd = hb.y - q.y;
-                            const double dzd = hb.z - q.z;
-                            const double r2 = sq3(dxd, dyd, dzd);
-                            if (r2 < cut2 && j != self) {
-                                const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
-                                const float r0 = __builtin_amdgcn_sqrtf((float)r2);
-                                float ri = __builtin_amdgcn_rcpf(r0);
-                                const float corr = (float)fma(-(double)r0, (double)r0, r2) * (0.5f * ri);
-                                const float r = r0 + corr;
-                                ri = ri * (2.0f - r * ri);
-                                const float *C = Cf + 6 * q.t;
-                                const int o = (r >= C[0]) ? 3 : 0;
-                                const float m = (C[o] - r0) - corr;
-                                const float mag = ((C[o + 1] - C[o + 2] * m) * m) * ri;
-                                fx += dx * mag;
-                                fy += dy * mag;
-                                fz += dz * mag;
-                            }
+                        int j = cb + sub;
+                        for (; j + tpp < ce; j += 2 * tpp) {
+                            const Cand q0 = Cw[j - cb];
+                            const Cand q1 = Cw[j + tpp - cb];
+                            force_term(hb, q0, j != self, cut2, Cf, fx, fy, fz);
+                            force_term(hb, q1, j + tpp != self, cut2, Cf, fx, fy, fz);
                         }
+                        if (j < ce) force_term(hb, Cw[j - cb], j != self, cut2, Cf, fx, fy, fz);
                     } else if (k2P) {
                         const float *Cf = sCf + 6 * hb.type * nT;
                         uint8_t *lst = &sList[k2P ? wave : 0][0][lane];
@@ -289,33 +326,8 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
                             for (int o = 1; o < kWave; o <<= 1) most = max(most, __shfl_xor(most, o));
                             // phase 2: exact FP64 test and the force law on the hits
-                            for (int kk = 0; kk < most; kk++) {
-                                if (kk < nhit) {
-                                    const Cand q = Cw[lst[kWave * kk]];
-                                    const double dxd = hb.x - q.x;
-                                    const double dyd = hb.y - q.y;
-                                    const double dzd = hb.z - q.z;
-                                    const double r2 = sq3(dxd, dyd, dzd);
-                                    if (r2 < cut2) {
-                                        const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
-                                        // r = r0 + corr: one Newton step against the exact FP64
-                                        // residual r^2 - r0^2, so m = C0 - r carries no FP32
-                                        // rounding of r^2 (near r_m the force is C1 m, C1 ~ 400).
-                                        const float r0 = __builtin_amdgcn_sqrtf((float)r2);
-                                        float ri = __builtin_amdgcn_rcpf(r0);
-                                        const float corr = (float)fma(-(double)r0, (double)r0, r2) * (0.5f * ri);
-                                        const float r = r0 + corr;
-                                        ri = ri * (2.0f - r * ri);
-                                        const float *C = Cf + 6 * q.t;
-                                        const int o = (r >= C[0]) ? 3 : 0;    // int(r/r_m)*3, clamped
-                                        const float m = (C[o] - r0) - corr;   // C[o] - r0 is exact
-                                        const float mag = ((C[o + 1] - C[o + 2] * m) * m) * ri;
-                                        fx += dx * mag;
-                                        fy += dy * mag;
-                                        fz += dz * mag;
-                                    }
-                                }
-                            }
+                            for (int kk = 0; kk < most; kk++)
+                                if (kk < nhit) force_term(hb, Cw[lst[kWave * kk]], true, cut2, Cf, fx, fy, fz);
                         }
                     } else if (!kF && act) {
                         for (int j = cb + sub; j < ce; j += tpp) {
@@ -327,6 +339,9 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                             const double sy = Y0 < 0 ? -Ly : (Y0 >= ny ? Ly : 0.0);
                             const double sz = Z0 < 0 ? -Lz : (Z0 >= nz ? Lz : 0.0);
                             const bool own = (c == 13) || ((fwd >> c) & 1u);
+                            // energies: each pair once, from the side that owns it (within
+                            // one cell, from the lower slot); counts and lists see both sides
+                            if ((MODE == M_U || MODE == M_DU) && (!own || (c == 13 && j < self))) continue;
                             double ox, oy, oz, px, py, pz;
                             if (own) {
                                 ox = hb.x; oy = hb.y; oz = hb.z;
@@ -339,12 +354,13 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                             if (MODE == M_U || MODE == M_DU) {
                                 const double *C = sCd + 6 * ((own ? hb.type : q.t) * nT + (own ? q.t : hb.type));
                                 if (MODE == M_U) {
-                                    if (r2 < cut2) accu += 0.5 * pair_u(r2, C);
+                                    if (r2 < cut2) accu += pair_u(r2, C);
                                 } else {
                                     const double oldP = (r2 < cut2) ? pair_u(r2, C) : 0.0;
-                                    const double r2n = sq3(ox * ax - px * ax, oy * ay - py * ay, oz * az - pz * az);
+                                    const double r2n = sq3(scaled_diff(ox, px, ax), scaled_diff(oy, py, ay),
+                                                           scaled_diff(oz, pz, az));
                                     const double newP = (r2n < cut2) ? pair_u(r2n, C) : 0.0;
-                                    accu += 0.5 * (oldP - newP);
+                                    accu += (oldP - newP);
                                 }
                             } else if (r2 < cut2) {
                                 cnt++;
@@ -387,25 +403,25 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         out.counts[me] = c;
                     }
                     wave_lds_sync();
+                } else if (MODE == M_U || MODE == M_DU) {
+                    // the home particle's share, its lanes combined in lane order
+                    redd[kF ? 0 : wave][lane] = accu;
+                    wave_lds_sync();
+                    if (act && sub == 0) {
+                        double e = 0.0;
+                        for (int t = 0; t < tpp; t++) e += redd[kF ? 0 : wave][lane + t];
+                        out.energy[s0 + h0 + hi] = e;
+                    }
+                    wave_lds_sync();
                 }
             }
         }
-    }
-    if (MODE == M_U || MODE == M_DU) {
-        // fixed-shape tree over the workgroup: the result does not depend on scheduling
-        redd[threadIdx.x] = accu;
-        __syncthreads();
-        for (int o = kWaves * kWave / 2; o > 0; o >>= 1) {
-            if ((int)threadIdx.x < o) redd[threadIdx.x] += redd[threadIdx.x + o];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out.partial[blockIdx.x] = redd[0];
     }
 }
 
 template <int MODE>
 void launch_cells(hipStream_t st, const Params &P, int chunk, const Bead *sbeads, const uint32_t *skey,
-                  const uint32_t *sidx, const int *cstart, const int *cend, const DevBox *box,
+                  const uint32_t *sidx, const int *cstart, const int *cend, const DevRep *reps,
                   const float *fconst, const double *uconst, const Out &out)
 {
     const int nb = cells_blocks(P.n, chunk);
@@ -417,7 +433,7 @@ void launch_cells(hipStream_t st, const Params &P, int chunk, const Bead *sbeads
     const double slack = 64.0 * P.cutoff * 1.5 * csmax * (1.0 / 8388608.0) + 1e-6 * cut2;
     const float cut2pre = (float)(cut2 + slack);
     hipLaunchKernelGGL(k_cells<MODE>, dim3(nb), dim3(kWaves * kWave), 0, st, P.n, P.nT, chunk, nb, cut2, cut2pre,
-                       sbeads, skey, sidx, cstart, cend, box, fconst, uconst, out);
+                       sbeads, skey, sidx, cstart, cend, reps, fconst, uconst, out);
 }
 
 }  // namespace
@@ -430,28 +446,28 @@ int cells_blocks(int n, int chunk)
 
 void launch_pair_force(hipStream_t st, const Params &P, int chunk, int variant, const Bead *sbeads,
                        const uint32_t *skey, const uint32_t *sidx, const int *cstart, const int *cend,
-                       const DevBox *box, const float *fconst, float4 *fpair)
+                       const DevRep *reps, const float *fconst, float4 *fpair)
 {
     Out o{fpair, nullptr, nullptr, nullptr, nullptr, 0};
     if (variant == 1)
-        launch_cells<M_FORCE>(st, P, chunk, sbeads, skey, sidx, cstart, cend, box, fconst, nullptr, o);
+        launch_cells<M_FORCE>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, fconst, nullptr, o);
     else
-        launch_cells<M_FORCE1>(st, P, chunk, sbeads, skey, sidx, cstart, cend, box, fconst, nullptr, o);
+        launch_cells<M_FORCE1>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, fconst, nullptr, o);
 }
 
 void launch_pair_gather(hipStream_t st, const Params &P, int chunk, int mode, const Bead *sbeads,
                         const uint32_t *skey, const uint32_t *sidx, const int *cstart, const int *cend,
-                        const DevBox *box, const double *uconst, double *partial, int *counts, int *pairs,
+                        const DevRep *reps, const double *uconst, double *energy, int *counts, int *pairs,
                         unsigned long long *npairs, long long cap)
 {
-    Out o{nullptr, partial, counts, pairs, npairs, cap};
+    Out o{nullptr, energy, counts, pairs, npairs, cap};
     switch (mode) {
-    case GATHER_U: launch_cells<M_U>(st, P, chunk, sbeads, skey, sidx, cstart, cend, box, nullptr, uconst, o); break;
-    case GATHER_DU: launch_cells<M_DU>(st, P, chunk, sbeads, skey, sidx, cstart, cend, box, nullptr, uconst, o); break;
+    case GATHER_U: launch_cells<M_U>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, nullptr, uconst, o); break;
+    case GATHER_DU: launch_cells<M_DU>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, nullptr, uconst, o); break;
     case GATHER_COUNT:
-        launch_cells<M_COUNT>(st, P, chunk, sbeads, skey, sidx, cstart, cend, box, nullptr, uconst, o);
+        launch_cells<M_COUNT>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, nullptr, uconst, o);
         break;
-    default: launch_cells<M_LIST>(st, P, chunk, sbeads, skey, sidx, cstart, cend, box, nullptr, uconst, o); break;
+    default: launch_cells<M_LIST>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, nullptr, uconst, o); break;
     }
 }
 

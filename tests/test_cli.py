@@ -93,7 +93,8 @@ def test_cli_job_matches_library(tmp_path):
     n = len(d["type"])
     assert kin[0, 1] == pytest.approx(1.5 * n * d["temperature"], rel=1e-6)
     import oracle
-    s0 = oracle.System.from_mpd(str(tmp_path / "job.mpd"))
+    s0 = oracle.System(d["pos"], d["type"], d["vel"], d["box"], d["fconst"], d["uconst"], d["n_types"],
+                       d["chain"], d["chain_const"], cutoff=d["cutoff"])
     assert pot[0, 1] == pytest.approx(s0.pair_potential() + s0.chain_potential(), rel=2e-6)
     np.testing.assert_allclose(size[:, 1:], np.tile(d["box"], (5, 1)), rtol=1e-6)
     np.testing.assert_allclose(temp[:, 1], d["temperature"])
@@ -110,7 +111,11 @@ def test_cli_job_matches_library(tmp_path):
     assert tmp["initialTime"] == out["initialTime"]
     o = out.system()
 
-    lib = sm.Solver.from_system(d)
+    # the library on the system as md_hip read it (the .mpd text keeps 15 digits, so the
+    # file's state is not bit-identical to the generator's in-memory doubles)
+    m0 = sm.Mpd.liposome(1234, 500, 3.45)
+    m0.write(str(tmp_path / "fresh"))
+    lib = sm.Solver.from_system(sm.Mpd.read(str(tmp_path / "fresh")).system())
     lib.init_forces()
     lib.run(0, 40)
     lib.step_first(40)
@@ -151,5 +156,19 @@ def test_cli_restart_and_mc(tmp_path):
     # first job: t=0,16,32,48,64; restart from 64 measures at 80, 96 (i != start)
     np.testing.assert_allclose(pot[:, 0] / m["deltaT"], [0, 16, 32, 48, 64, 80, 96], atol=1e-9)
     assert np.isfinite(pot[:, 1]).all()
-    # the energy per bead stays in the membrane range across the restart
-    assert abs(pot[-1, 1] - pot[4, 1]) < 0.05 * abs(pot[4, 1])
+    # the restarted run continues the uninterrupted one: same job in one go to t = 96
+    # (the noise after the restart is drawn afresh, as the reference's is, so the
+    # comparison is statistical: the vesicle is still relaxing, alike in both runs)
+    _make(tmp_path, "one", steps=96, store=32, measure=16, dlxy=0.05, tension=1.0)
+    print("case: md_hip MC 0..96 in one job", flush=True)
+    r = _run(["one"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    one = _table(tmp_path / "potential_one.dat")
+    np.testing.assert_allclose(one[:5], pot[:5], rtol=1e-12)    # identical up to the store
+    np.testing.assert_allclose(one[5:, 1], pot[5:, 1], rtol=0.02)
+    # the kinetic energy after the restart follows the uninterrupted run (the generated
+    # vesicle is still releasing potential energy into heat at this point, alike in both)
+    kin = _table(tmp_path / "kinetic_mc.dat")
+    kin1 = _table(tmp_path / "kinetic_one.dat")
+    np.testing.assert_allclose(kin1[:5], kin[:5], rtol=1e-12)
+    np.testing.assert_allclose(kin1[5:, 1], kin[5:, 1], rtol=0.1)

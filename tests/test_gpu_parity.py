@@ -36,6 +36,7 @@ def system(name):
             "L5000": lambda: sm.Mpd.liposome(1234, 5000, 3.45),     # 15,000 beads
             "B4000": lambda: sm.Mpd.bilayer(1234, 4000, 3.45),      # 12,000 beads
             "L80000": lambda: sm.Mpd.liposome(1234, 80000, 3.45),   # 240,000 beads
+            "B1M": lambda: sm.Mpd.bilayer(1234, 1000000, 3.45),     # 2,999,094 beads
         }[name]
         _cache[name] = gen().system()
     return _cache[name]
@@ -57,7 +58,7 @@ def force_errors(f, ref):
     return per, l2
 
 
-@pytest.mark.parametrize("name", ["L500", "B400", "L5000", "B4000", "L80000"])
+@pytest.mark.parametrize("name", ["L500", "B400", "L5000", "B4000", "L80000", "B1M"])
 def test_cell_keys_bit_exact(name):
     d = system(name)
     print(f"case cell_keys {name} N={len(d['type'])}", flush=True)
@@ -76,7 +77,7 @@ def test_pair_set_bit_exact(name):
     assert set(map(tuple, ours.tolist())) == set(map(tuple, ref.tolist()))
 
 
-@pytest.mark.parametrize("name", ["L5000", "L80000"])
+@pytest.mark.parametrize("name", ["L5000", "L80000", "B1M"])
 def test_neighbour_counts_bit_exact(name):
     d = system(name)
     print(f"case counts {name}", flush=True)
@@ -87,7 +88,7 @@ def test_neighbour_counts_bit_exact(name):
     assert total == rc.sum() // 2
 
 
-@pytest.mark.parametrize("name", ["L500", "B400", "L5000", "B4000", "L80000"])
+@pytest.mark.parametrize("name", ["L500", "B400", "L5000", "B4000", "L80000", "B1M"])
 def test_step0_pair_forces_fp32_vs_fp64(name):
     d = system(name)
     print(f"case forces {name}", flush=True)
@@ -116,7 +117,7 @@ def test_chain_forces_on_deformed_chains():
     assert l2 < 1e-6 and per.max() < 1e-5
 
 
-@pytest.mark.parametrize("name", ["L500", "B400", "L5000", "B4000"])
+@pytest.mark.parametrize("name", ["L500", "B400", "L5000", "B4000", "B1M"])
 def test_energies_fp64(name):
     d = system(name)
     s = sm.Solver.from_system(d)
@@ -137,7 +138,7 @@ def test_cfg1_known_answer():
     assert K == pytest.approx(1.5 * 15000 * 3.0, rel=1e-6)
 
 
-@pytest.mark.parametrize("name", ["B400", "B4000", "L5000"])
+@pytest.mark.parametrize("name", ["B400", "B4000", "L5000", "B1M"])
 def test_potential_delta_fp64(name):
     d = system(name)
     s = sm.Solver.from_system(d)
