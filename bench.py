@@ -203,9 +203,13 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=1000)
     ap.add_argument("--calib-copy", action="store_true")
+    ap.add_argument("--batched-only", action="store_true", help="only the replica sweep (for profiling it)")
     args = ap.parse_args()
     if args.calib_copy:
         calib_copy()
+        return
+    if args.batched_only:
+        print(json.dumps({"batched": bench_batched(args.batched_steps, 100, 200)}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the solver has no multi-GPU path "

@@ -93,7 +93,11 @@ struct sm_ctx {
     // device state
     Bead *beads = nullptr, *sbeads = nullptr;
     float4 *vel = nullptr, *acc = nullptr, *fpair = nullptr, *fscratch = nullptr;
-    uint32_t *k0 = nullptr, *v0 = nullptr, *k1 = nullptr, *v1 = nullptr, *hist = nullptr;
+    uint32_t *k0 = nullptr, *v0 = nullptr, *k1 = nullptr, *v1 = nullptr;
+    uint32_t *hist[2] = {nullptr, nullptr};   // sort scratch, double-buffered
+    size_t hist_words = 0;
+    int sort_parity = 0;                      // scratch the next sort uses
+    bool hist_pending = false;                // its histograms hold keys not sorted yet
     int *cstart = nullptr, *cend = nullptr;
     long long capacity = 0;   // cells in the dense table, all replicas
     int sort_bits = 0;
@@ -117,9 +121,9 @@ struct sm_ctx {
     int sorted_sel = 0;   // which sort buffer pair holds the sorted result
     // captured 8-step graph
     bool use_graph = false;
-    bool graph_valid = false;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
+    // one graph per sort-scratch parity at the start of the block (8 sorts return to it)
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
     // profiling
     bool prof = false;
     struct Rec { int cls; hipEvent_t a, b; };
@@ -140,7 +144,7 @@ struct sm_ctx {
         for (auto &r : recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
         for (auto e : pool) (void)hipEventDestroy(e);
         dfree(beads); dfree(sbeads); dfree(vel); dfree(acc); dfree(fpair); dfree(fscratch);
-        dfree(k0); dfree(v0); dfree(k1); dfree(v1); dfree(hist); dfree(cstart); dfree(cend);
+        dfree(k0); dfree(v0); dfree(k1); dfree(v1); dfree(hist[0]); dfree(hist[1]); dfree(cstart); dfree(cend);
         dfree(reps); dfree(glob); dfree(fconstF); dfree(uconstD); dfree(chainPerBead); dfree(chainList);
         dfree(cconstD); dfree(ebead); dfree(echain); dfree(dres); dfree(dcounts); dfree(dpairs); dfree(dnp);
         dfree(adjF); dfree(adjDF); dfree(adjDU);
@@ -151,11 +155,12 @@ struct sm_ctx {
 
     void drop_graph()
     {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (graph) (void)hipGraphDestroy(graph);
-        gexec = nullptr;
-        graph = nullptr;
-        graph_valid = false;
+        for (int p = 0; p < 2; p++) {
+            if (gexec[p]) (void)hipGraphExecDestroy(gexec[p]);
+            if (graph[p]) (void)hipGraphDestroy(graph[p]);
+            gexec[p] = nullptr;
+            graph[p] = nullptr;
+        }
     }
 
     // ---- profiling helpers ----
@@ -278,18 +283,31 @@ struct sm_ctx {
         sort_bits = bits;
     }
 
-    // keys (optionally after Verlet::first), sort, reorder + dense cell table
-    void first_and_keys(int integrate, long long step_set)
+    // keys (optionally after Verlet::first) with the next sort's digit histograms, sort,
+    // reorder + dense cell table (clearing the other sort scratch)
+    void first_and_keys(int integrate, long long step_set, bool for_sort = true)
     {
-        timed(K_FIRST, [&] { launch_first(st, P, beads, vel, acc, reps, glob, k0, v0, integrate, step_set); });
+        uint32_t *gh = for_sort ? hist[sort_parity] : nullptr;
+        // keys computed twice before a sort (sm_step_first called twice): start over
+        if (for_sort && hist_pending) launch_fill_u32(st, gh, 4 * 256, 0u);
+        hist_pending = hist_pending || for_sort;
+        const int passes = sort_passes(sort_bits);
+        timed(K_FIRST, [&] {
+            launch_first(st, P, beads, vel, acc, reps, glob, k0, v0, integrate, step_set, gh, passes);
+        });
     }
     void sort_and_reorder()
     {
+        const int p = sort_parity;
         timed(K_SORT, [&] {
-            SortBuffers sb{k0, v0, k1, v1, hist, &glob->flags, n, 0};
-            sorted_sel = radix_sort(st, sb, sort_bits);
+            SortBuffers sb{k0, v0, k1, v1, {hist[0], hist[1]}, &glob->flags, n, 0};
+            sorted_sel = radix_sort(st, sb, sort_bits, p);
         });
-        timed(K_REORDER, [&] { launch_reorder(st, n, skey(), sidx(), beads, sbeads, cstart, cend); });
+        timed(K_REORDER, [&] {
+            launch_reorder(st, n, skey(), sidx(), beads, sbeads, cstart, cend, hist[1 - p], hist_words);
+        });
+        sort_parity = 1 - p;
+        hist_pending = false;
     }
     void build(int integrate, long long step_set)
     {
@@ -369,24 +387,24 @@ struct sm_ctx {
     // has one) falls on the block's last step.
     void graph_block(long long s)
     {
-        if (!graph_valid) {
-            drop_graph();
+        const int p = sort_parity;   // 8 sorts later the parity is p again
+        if (!gexec[p]) {
             HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             for (int k = 0; k < 8; k++) {
                 step_first(kStepAdvance);
                 step_finish(false, k == 7 && any_mc);
             }
-            hipError_t e = hipStreamEndCapture(st, &graph);
+            hipError_t e = hipStreamEndCapture(st, &graph[p]);
             if (e != hipSuccess) {
-                graph = nullptr;
+                graph[p] = nullptr;
                 HIPCHK(e);
             }
-            HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-            graph_valid = true;
+            HIPCHK(hipGraphInstantiate(&gexec[p], graph[p], nullptr, nullptr, 0));
         }
+        sort_parity = p;   // capture only recorded the toggles
         // the block advances from step s - 1
         launch_set_step(st, glob, s - 1);
-        HIPCHK(hipGraphLaunch(gexec, st));
+        HIPCHK(hipGraphLaunch(gexec[p], st));
     }
 
     void run(long long first, long long nsteps)
@@ -584,7 +602,11 @@ sm_ctx *create(const sm_system *s, int nrep)
         c->v0 = dalloc<uint32_t>(n);
         c->k1 = dalloc<uint32_t>(n);
         c->v1 = dalloc<uint32_t>(n);
-        c->hist = dalloc<uint32_t>(sort_scratch_words(n, 4));
+        c->hist_words = sort_scratch_words(n, 4);
+        for (int p = 0; p < 2; p++) {
+            c->hist[p] = dalloc<uint32_t>(c->hist_words);
+            HIPCHK(hipMemsetAsync(c->hist[p], 0, sizeof(uint32_t) * c->hist_words, c->st));
+        }
         c->reps = dalloc<DevRep>(nrep);
         c->glob = dalloc<DevGlobal>(1);
         c->fconstF = dalloc<float>(fF.size());
@@ -916,7 +938,7 @@ int sm_cell_keys(sm_ctx *c, int32_t *keys)
 {
     return guarded([&] {
         if (!c || !keys) invalid("null argument");
-        c->first_and_keys(0, kStepKeep);
+        c->first_and_keys(0, kStepKeep, false);   // no sort follows: leave its histograms alone
         c->check_launch();
         std::vector<uint32_t> h(c->n);
         HIPCHK(hipMemcpyAsync(h.data(), c->k0, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost, c->st));

@@ -93,12 +93,15 @@ struct SegTiles {
 void launch_seg_sum(hipStream_t st, const SegTiles &S, const double *x, double *out, int stride);
 
 // ---- launchers (sm_kernels.hip) ---------------------------------------------------------
-// step_set >= 0: the device step counter is set to step_set; < 0: it is advanced by one
-// (the form a captured graph replays).
+// step_set >= 0: the device step counter is set to step_set; -1: it is advanced by one
+// (the form a captured graph replays); <= -2: left as it is.  If ghist != NULL the digit
+// histograms of the first `passes` 8-bit digits of the keys are added to it.
 void launch_first(hipStream_t st, const Params &P, Bead *beads, float4 *vel, const float4 *acc, const DevRep *reps,
-                  DevGlobal *glob, uint32_t *key, uint32_t *idx, int integrate, long long step_set);
+                  DevGlobal *glob, uint32_t *key, uint32_t *idx, int integrate, long long step_set, uint32_t *ghist,
+                  int passes);
+// also zeroes clear[0 .. clear_words) (the next sort's scratch)
 void launch_reorder(hipStream_t st, int n, const uint32_t *skey, const uint32_t *sidx, const Bead *beads,
-                    Bead *sbeads, int *cstart, int *cend);
+                    Bead *sbeads, int *cstart, int *cend, uint32_t *clear, size_t clear_words);
 // variant 0: one phase (FP64 test, force law under the branch); 1: FP32 screen into a
 // per-lane hit list, then FP64 test + force law on the hits
 void launch_pair_force(hipStream_t st, const Params &P, int chunk, int variant, const Bead *sbeads,
@@ -136,14 +139,16 @@ void launch_set_step(hipStream_t st, DevGlobal *glob, long long step);
 // ---- radix sort (sm_sort.hip) ---------------------------------------------------------
 struct SortBuffers {
     uint32_t *k0, *v0, *k1, *v1;
-    uint32_t *scratch;   // sort_scratch_words(n, 4) words
-    int *errflag;        // receives FLAG_SORT_TIMEOUT
+    uint32_t *scratch[2];   // sort_scratch_words(n, 4) words each, double-buffered
+    int *errflag;           // receives FLAG_SORT_TIMEOUT
     int n;
     int tiles;
 };
 size_t sort_scratch_words(int n, int max_passes);
-// Stable LSD sort of (k0,v0) by the low `bits` bits; returns which buffer pair holds the
-// result (0 -> k0/v0, 1 -> k1/v1).
-int radix_sort(hipStream_t st, SortBuffers &b, int bits);
+int sort_passes(int bits);
+// Stable LSD sort of (k0,v0) by the low `bits` bits using scratch[parity], whose digit
+// histograms k_first has filled and whose other words are zero; returns which buffer pair
+// holds the result (0 -> k0/v0, 1 -> k1/v1).
+int radix_sort(hipStream_t st, SortBuffers &b, int bits, int parity);
 
 }  // namespace smdev

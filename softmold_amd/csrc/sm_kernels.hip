@@ -4,7 +4,7 @@
 // deterministic segmented FP64 sums, and the device-side Monte-Carlo box move.  Every
 // kernel serves all replicas of a context in one launch; a bead finds its replica's box,
 // thermostat and MC state through Bead::rep.
-#include "sm_internal.hpp"
+#include "sm_sort_dev.hpp"
 
 namespace smdev {
 
@@ -55,15 +55,11 @@ __device__ inline bool in_box(double p, double L) { return p >= 0.0 && p <= L; }
 // ---------------------------------------------------------------------------------------
 // Verlet::first (verlet.h:289-356) + cell key (cellOpt.h:530-556)
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_first(int n, Bead *__restrict__ beads, float4 *__restrict__ vel,
-                                                  const float4 *__restrict__ acc, const DevRep *__restrict__ reps,
-                                                  DevGlobal *__restrict__ glob, uint32_t *__restrict__ key,
-                                                  uint32_t *__restrict__ idx, float halfdt, double dt, int integrate,
-                                                  long long step_set)
+// Verlet::first of bead i and its cell key
+__device__ inline uint32_t first_one(int i, Bead *__restrict__ beads, float4 *__restrict__ vel,
+                                     const float4 *__restrict__ acc, const DevRep *__restrict__ reps,
+                                     DevGlobal *__restrict__ glob, float halfdt, double dt, int integrate)
 {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i == 0) glob->step = step_set >= 0 ? step_set : glob->step + 1;   // no other reader in this launch
-    if (i >= n) return;
     Bead b = beads[i];
     const DevRep &R = reps[b.rep];
     const double L0 = R.L[0], L1 = R.L[1], L2 = R.L[2];
@@ -90,19 +86,48 @@ __global__ __launch_bounds__(kBlock) void k_first(int n, Bead *__restrict__ bead
     const int cx = cell_axis(b.x, R.cs[0], nx);
     const int cy = cell_axis(b.y, R.cs[1], ny);
     const int cz = cell_axis(b.z, R.cs[2], nz);
-    key[i] = (uint32_t)(R.cellbase + (long long)(cx + cy * nx + cz * nx * ny));
-    idx[i] = (uint32_t)i;
+    return (uint32_t)(R.cellbase + (long long)(cx + cy * nx + cz * nx * ny));
+}
+
+__global__ __launch_bounds__(kBlock) void k_first(int n, Bead *__restrict__ beads, float4 *__restrict__ vel,
+                                                  const float4 *__restrict__ acc, const DevRep *__restrict__ reps,
+                                                  DevGlobal *__restrict__ glob, uint32_t *__restrict__ key,
+                                                  uint32_t *__restrict__ idx, float halfdt, double dt, int integrate,
+                                                  long long step_set, uint32_t *__restrict__ ghist, int passes)
+{
+    __shared__ uint32_t h[4][256];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i == 0 && step_set >= -1) glob->step = step_set >= 0 ? step_set : glob->step + 1;   // no other reader here
+    if (ghist) {
+        for (int p = 0; p < passes; p++) h[p][threadIdx.x] = 0u;
+        __syncthreads();
+    }
+    uint32_t k = 0u;
+    if (i < n) k = first_one(i, beads, vel, acc, reps, glob, halfdt, dt, integrate);
+    if (ghist) {
+        // the sort's digit histograms while the key is in a register (all lanes: ballots)
+        hist_add(h, k, i < n, passes);
+        __syncthreads();
+        hist_flush(h, ghist, passes);
+    }
+    if (i < n) {
+        key[i] = k;
+        idx[i] = (uint32_t)i;
+    }
 }
 
 // ---------------------------------------------------------------------------------------
-// reorder into cell order; cell runs written into the dense (never cleared) table
+// reorder into cell order; cell runs written into the dense (never cleared) table; the
+// next sort's scratch zeroed on the side
 // ---------------------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_reorder(int n, const uint32_t *__restrict__ skey,
                                                     const uint32_t *__restrict__ sidx, const Bead *__restrict__ beads,
                                                     Bead *__restrict__ sbeads, int *__restrict__ cstart,
-                                                    int *__restrict__ cend)
+                                                    int *__restrict__ cend, uint32_t *__restrict__ clear,
+                                                    size_t clear_words)
 {
     const int s = blockIdx.x * kBlock + threadIdx.x;
+    for (size_t w = (size_t)s; w < clear_words; w += (size_t)gridDim.x * kBlock) clear[w] = 0u;
     if (s >= n) return;
     const uint32_t k = skey[s];
     sbeads[s] = beads[sidx[s]];
@@ -459,16 +484,18 @@ inline int blocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 // launchers
 // ---------------------------------------------------------------------------------------
 void launch_first(hipStream_t st, const Params &P, Bead *beads, float4 *vel, const float4 *acc, const DevRep *reps,
-                  DevGlobal *glob, uint32_t *key, uint32_t *idx, int integrate, long long step_set)
+                  DevGlobal *glob, uint32_t *key, uint32_t *idx, int integrate, long long step_set, uint32_t *ghist,
+                  int passes)
 {
     hipLaunchKernelGGL(k_first, dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, beads, vel, acc, reps, glob, key, idx,
-                       (float)(0.5 * P.dt), P.dt, integrate, step_set);
+                       (float)(0.5 * P.dt), P.dt, integrate, step_set, ghist, passes);
 }
 
 void launch_reorder(hipStream_t st, int n, const uint32_t *skey, const uint32_t *sidx, const Bead *beads,
-                    Bead *sbeads, int *cstart, int *cend)
+                    Bead *sbeads, int *cstart, int *cend, uint32_t *clear, size_t clear_words)
 {
-    hipLaunchKernelGGL(k_reorder, dim3(blocks(n)), dim3(kBlock), 0, st, n, skey, sidx, beads, sbeads, cstart, cend);
+    hipLaunchKernelGGL(k_reorder, dim3(blocks(n)), dim3(kBlock), 0, st, n, skey, sidx, beads, sbeads, cstart, cend,
+                       clear, clear_words);
 }
 
 void launch_bonded_kick(hipStream_t st, const Params &P, const Bead *beads, const int2 *chain, const double4 *cconst,

@@ -2,19 +2,22 @@
 //
 // The reference sorts with thrust::sort_by_key (include/cuda/cell.h:254-260) and its CPU
 // path keeps unsorted linked lists (include/algorithms/cellOpt.h:572-585).  Here: 8-bit
-// digits, ceil(bits/8) passes, one kernel per pass ("onesweep"):
-//   clear    zero the digit histograms, tile counters and look-back states
-//   ghist    digit histograms of every pass at once (order independent)
-//   pass     a workgroup takes the next tile id from a counter (so every lower tile is
-//            already running), ranks its items with wave64 ballots, publishes its
-//            per-digit counts, accumulates the counts of the tiles before it by
-//            decoupled look-back, and scatters.
+// digits, ceil(bits/8) passes, one kernel per pass ("onesweep"): a workgroup takes the
+// next tile id from a counter (so every lower tile is already running), ranks its items
+// with wave64 ballots, publishes its per-digit counts, accumulates the counts of the
+// tiles before it by decoupled look-back, and scatters.
+//
+// No kernel of its own clears or counts: the digit histograms of every pass are built by
+// the kernel that computes the keys (k_first), and the sort's scratch (histograms, tile
+// counters, look-back states) is double-buffered -- while one sort uses scratch[p], the
+// reorder kernel after it clears scratch[1-p] for the next sort.
+//
 // Counting is wave-aggregated: the lanes holding one digit are found with 8 ballots and
 // their lowest lane writes the count once, so a digit shared by a whole wave (the high
 // digits of a compact system) costs one LDS write instead of 64 contended atomics.
 // The look-back spin is bounded: on timeout the kernel records FLAG_SORT_TIMEOUT in the
 // caller's flag word, clamps every write in bounds, and the host reports an error.
-#include "sm_internal.hpp"
+#include "sm_sort_dev.hpp"
 
 namespace smdev {
 
@@ -26,55 +29,6 @@ constexpr int kRounds = 4;
 constexpr int kTile = kThreads * kRounds;
 constexpr uint32_t kFlagAgg = 1u << 30, kFlagIncl = 2u << 30, kCountMask = (1u << 30) - 1;
 constexpr int kSpinLimit = 1 << 20;
-
-// lanes of the wave holding the same 8-bit digit as this lane (invalid lanes excluded)
-__device__ inline unsigned long long match_digit(uint32_t d, bool valid)
-{
-    unsigned long long peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < 8; b++) {
-        const bool bit = (d >> b) & 1u;
-        const unsigned long long m = __ballot(bit);
-        peers &= bit ? m : ~m;
-    }
-    return peers;
-}
-
-__device__ inline unsigned long long lanes_below()
-{
-    const int lane = threadIdx.x & (kWave - 1);
-    return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-}
-
-__global__ __launch_bounds__(256) void k_clear(uint32_t *__restrict__ a, size_t count)
-{
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < count) a[i] = 0u;
-}
-
-__global__ __launch_bounds__(kThreads) void k_ghist(int n, const uint32_t *__restrict__ keys, int passes,
-                                                    uint32_t *__restrict__ ghist)
-{
-    __shared__ uint32_t h[4][256];
-    for (int p = 0; p < 4; p++) h[p][threadIdx.x] = 0;
-    __syncthreads();
-    const unsigned long long lt = lanes_below();
-    const int stride = gridDim.x * kThreads;
-    const int nloop = (n + stride - 1) / stride;   // uniform trip count: ballots need every lane
-    for (int it = 0; it < nloop; it++) {
-        const int i = it * stride + blockIdx.x * kThreads + threadIdx.x;
-        const bool valid = i < n;
-        const uint32_t k = valid ? keys[i] : 0u;
-        for (int p = 0; p < passes; p++) {
-            const uint32_t d = (k >> (8 * p)) & 255u;
-            const unsigned long long peers = match_digit(d, valid);
-            if (valid && (peers & lt) == 0ull) atomicAdd(&h[p][d], (uint32_t)__popcll(peers));
-        }
-    }
-    __syncthreads();
-    for (int p = 0; p < passes; p++)
-        if (h[p][threadIdx.x]) atomicAdd(&ghist[p * 256 + threadIdx.x], h[p][threadIdx.x]);
-}
 
 __global__ __launch_bounds__(kThreads) void k_onesweep(int n, const uint32_t *__restrict__ kin,
                                                        const uint32_t *__restrict__ vin,
@@ -173,6 +127,12 @@ __global__ __launch_bounds__(kThreads) void k_onesweep(int n, const uint32_t *__
 
 }  // namespace
 
+int sort_passes(int bits)
+{
+    int passes = (bits + 7) / 8;
+    return passes < 1 ? 1 : (passes > 4 ? 4 : passes);
+}
+
 size_t sort_scratch_words(int n, int max_passes)
 {
     const size_t tiles = (size_t)((n + kTile - 1) / kTile) + 1;
@@ -180,22 +140,16 @@ size_t sort_scratch_words(int n, int max_passes)
     return 4 * 256 + 4 + (size_t)max_passes * tiles * 256;
 }
 
-int radix_sort(hipStream_t st, SortBuffers &b, int bits)
+int radix_sort(hipStream_t st, SortBuffers &b, int bits, int parity)
 {
     const int n = b.n;
     if (n <= 1) return 0;
-    int passes = (bits + 7) / 8;
-    if (passes < 1) passes = 1;
-    if (passes > 4) passes = 4;
+    const int passes = sort_passes(bits);
     const int tiles = (n + kTile - 1) / kTile;
     b.tiles = tiles;
-    uint32_t *ghist = b.scratch;
+    uint32_t *ghist = b.scratch[parity];
     uint32_t *tilectr = ghist + 4 * 256;
     uint32_t *status = tilectr + 4;
-    const size_t words = 4 * 256 + 4 + (size_t)passes * tiles * 256;
-    hipLaunchKernelGGL(k_clear, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, st, b.scratch, words);
-    const int gb = min(512, (n + kThreads - 1) / kThreads);
-    hipLaunchKernelGGL(k_ghist, dim3(gb), dim3(kThreads), 0, st, n, b.k0, passes, ghist);
     uint32_t *ki = b.k0, *vi = b.v0, *ko = b.k1, *vo = b.v1;
     for (int p = 0; p < passes; p++) {
         hipLaunchKernelGGL(k_onesweep, dim3(tiles), dim3(kThreads), 0, st, n, ki, vi, ko, vo, 8 * p, ghist + p * 256,

@@ -1,0 +1,50 @@
+// Device helpers of the cell-key radix sort shared by the sort kernels (sm_sort.hip) and
+// the kernel that computes the keys (k_first in sm_kernels.hip), which also builds the
+// sort's digit histograms while the keys are in registers.
+#pragma once
+#include "sm_internal.hpp"
+
+namespace smdev {
+
+// lanes of the wave holding the same 8-bit digit as this lane (invalid lanes excluded)
+__device__ inline unsigned long long match_digit(uint32_t d, bool valid)
+{
+    unsigned long long peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        const bool bit = (d >> b) & 1u;
+        const unsigned long long m = __ballot(bit);
+        peers &= bit ? m : ~m;
+    }
+    return peers;
+}
+
+__device__ inline unsigned long long lanes_below()
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Adds the digits of one key per thread (all `passes` 8-bit digits) to a block histogram
+// h[4][256] in LDS, wave-aggregated: the lowest lane holding a digit adds its wave's count,
+// so a digit shared by a whole wave costs one LDS atomic.  Every lane of the block must
+// call it (ballots).
+__device__ inline void hist_add(uint32_t (*h)[256], uint32_t key, bool valid, int passes)
+{
+    const unsigned long long lt = lanes_below();
+    for (int p = 0; p < passes; p++) {
+        const uint32_t d = (key >> (8 * p)) & 255u;
+        const unsigned long long peers = match_digit(d, valid);
+        if (valid && (peers & lt) == 0ull) atomicAdd(&h[p][d], (uint32_t)__popcll(peers));
+    }
+}
+
+// Flushes a block histogram into the global digit histograms ghist[passes][256].
+// Needs blockDim.x == 256 and a __syncthreads() before the call.
+__device__ inline void hist_flush(uint32_t (*h)[256], uint32_t *ghist, int passes)
+{
+    for (int p = 0; p < passes; p++)
+        if (h[p][threadIdx.x]) atomicAdd(&ghist[p * 256 + threadIdx.x], h[p][threadIdx.x]);
+}
+
+}  // namespace smdev

@@ -56,11 +56,27 @@ def _oracle_J(d, theta, nsteps):
     return O.MD(s, rng_mode=1).objective(nsteps)
 
 
+def _fd64(d, S, scale):
+    """Central differences of the FP64 oracle's J with steps STEP * scale."""
+    fd = np.zeros(4)
+    for p in range(4):
+        h = STEP[p] * scale
+        a, b = THETA0.copy(), THETA0.copy()
+        a[p] += h
+        b[p] -= h
+        fd[p] = (_oracle_J(d, a, S) - _oracle_J(d, b, S)) / (2 * h)
+    return fd
+
+
 @pytest.mark.gpu
-def test_adjoint_gradient_matches_differences():
+def test_adjoint_gradient_matches_fp64_reference():
+    """40 steps: the trajectory is chaotic enough that central differences need small
+    steps to stay in the linear regime (at STEP they are off by up to 17 %, converging to
+    the adjoint as the step shrinks), so the FP64 oracle, free of FP32 noise, supplies
+    the reference gradient with steps STEP * 1e-4."""
     if not gpu_available():
         pytest.skip("no GPU")
-    from softmold_amd.adjoint import get_params, objective, set_params
+    from softmold_amd.adjoint import get_params, objective
 
     d = _system()
     S, k = 40, 10
@@ -74,25 +90,40 @@ def test_adjoint_gradient_matches_differences():
         assert np.array_equal(x, y)
     # the same J from the FP64 oracle along its own trajectory
     J64 = _oracle_J(d, THETA0, S)
-    print(f"J gpu {J:.10g} oracle {J64:.10g}; grad {g}")
+    fd64 = _fd64(d, S, 1e-4)
+    import adjoint_ref as AR
+    gref = AR.adjoint_gradient(d, THETA0, PAIR, S)
+    print(f"J gpu {J:.10g} oracle {J64:.10g}\nadjoint {g}\nFP64 central differences {fd64}\n"
+          f"FP64 reference gradient {gref}")
     assert J == pytest.approx(J64, rel=1e-6)
+    np.testing.assert_allclose(g, fd64, rtol=1e-3)
+    np.testing.assert_allclose(g, gref, rtol=1e-3)
 
-    fd, fd64 = np.zeros(4), np.zeros(4)
+
+@pytest.mark.gpu
+def test_adjoint_gradient_matches_gpu_differences():
+    """3 steps, in the linear regime at STEP: the adjoint against central differences of
+    the GPU's own FP32-forward J, and of the oracle's."""
+    if not gpu_available():
+        pytest.skip("no GPU")
+    from softmold_amd.adjoint import objective, set_params
+
+    d = _system()
+    S = 3
+    s = sm.Solver.from_system(d)
+    J, g = objective(s, PAIR, S, 1)
+    fd = np.zeros(4)
     for p in range(4):
         Js = []
         for sgn in (1, -1):
             th = THETA0.copy()
             th[p] += sgn * STEP[p]
             set_params(s, PAIR, th)
-            Js.append(objective(s, PAIR, S, k, grad=False)[0])
+            Js.append(objective(s, PAIR, S, 1, grad=False)[0])
         fd[p] = (Js[0] - Js[1]) / (2 * STEP[p])
-        th_p, th_m = THETA0.copy(), THETA0.copy()
-        th_p[p] += STEP[p]
-        th_m[p] -= STEP[p]
-        fd64[p] = (_oracle_J(d, th_p, S) - _oracle_J(d, th_m, S)) / (2 * STEP[p])
     set_params(s, PAIR, THETA0)
-    print("central differences, GPU FP32 forward:", fd)
-    print("central differences, FP64 oracle:     ", fd64)
+    fd64 = _fd64(d, S, 1.0)
+    print(f"adjoint {g}\nGPU central differences {fd}\nFP64 central differences {fd64}")
     np.testing.assert_allclose(g, fd, rtol=1e-3)
     np.testing.assert_allclose(g, fd64, rtol=1e-3)
 
@@ -125,3 +156,17 @@ def test_autograd_function():
     Jr, gr = objective(s, PAIR, 16, 4)
     assert float(J) == Jr
     np.testing.assert_array_equal(theta.grad.numpy(), 2.0 * gr)
+
+
+def test_fp64_reference_gradient_matches_fine_differences():
+    """The FP64 CPU reference gradient (oracle/adjoint_ref.py: the reverse recursion on the
+    oracle's trajectory with finite-difference Hessian-vector products) against central
+    differences of the oracle's J with small steps."""
+    import adjoint_ref as AR
+
+    d = _system(300)
+    S = 20
+    g = AR.adjoint_gradient(d, THETA0, PAIR, S)
+    fd = _fd64(d, S, 1e-4)
+    print("reference", g, "differences", fd)
+    np.testing.assert_allclose(g, fd, rtol=1e-4)

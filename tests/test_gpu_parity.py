@@ -99,8 +99,10 @@ def test_step0_pair_forces_fp32_vs_fp64(name):
     print(f"{name}: per-particle max {per.max():.3e} p99.9 {np.quantile(per, 0.999):.3e} L2 {l2:.3e}")
     assert l2 < FORCE_TOL_L2
     assert per.max() < FORCE_TOL
-    # Newton's third law holds to FP32 rounding
-    assert np.linalg.norm(fp.sum(0)) < 1e-5 * np.sqrt(len(fp)) * np.abs(ref).max()
+    # Newton's third law holds to FP32 rounding: each pair force is exactly antisymmetric
+    # (r and the magnitude depend on r^2 only), so what is left is the FP32 rounding of the
+    # per-particle sums, at most a few ulps of sum |F_i| in total
+    assert np.linalg.norm(fp.sum(0)) < 2.0 * 2.0 ** -24 * np.abs(fp).sum()
 
 
 def test_chain_forces_on_deformed_chains():
