@@ -289,7 +289,7 @@ struct sm_ctx {
     {
         uint32_t *gh = for_sort ? hist[sort_parity] : nullptr;
         // keys computed twice before a sort (sm_step_first called twice): start over
-        if (for_sort && hist_pending) launch_fill_u32(st, gh, 4 * 256, 0u);
+        if (for_sort && hist_pending) launch_fill_u32(st, gh, sort_hist_words(), 0u);
         hist_pending = hist_pending || for_sort;
         const int passes = sort_passes(sort_bits);
         timed(K_FIRST, [&] {

@@ -102,8 +102,9 @@ void launch_first(hipStream_t st, const Params &P, Bead *beads, float4 *vel, con
 // also zeroes clear[0 .. clear_words) (the next sort's scratch)
 void launch_reorder(hipStream_t st, int n, const uint32_t *skey, const uint32_t *sidx, const Bead *beads,
                     Bead *sbeads, int *cstart, int *cend, uint32_t *clear, size_t clear_words);
-// variant 0: one phase (FP64 test, force law under the branch); 1: FP32 screen into a
-// per-lane hit list, then FP64 test + force law on the hits
+// variant 0: one phase (FP64 test, force law under the branch) over per-half-cell culled
+// candidate lists; 1: FP32 screen into a per-lane hit list, then FP64 test + force law on
+// the hits
 void launch_pair_force(hipStream_t st, const Params &P, int chunk, int variant, const Bead *sbeads,
                        const uint32_t *skey, const uint32_t *sidx, const int *cstart, const int *cend,
                        const DevRep *reps, const float *fconst, float4 *fpair);
@@ -145,6 +146,7 @@ struct SortBuffers {
     int tiles;
 };
 size_t sort_scratch_words(int n, int max_passes);
+size_t sort_hist_words();   // the digit histograms at the start of the scratch
 int sort_passes(int bits);
 // Stable LSD sort of (k0,v0) by the low `bits` bits using scratch[parity], whose digit
 // histograms k_first has filled and whose other words are zero; returns which buffer pair

@@ -96,7 +96,8 @@ __global__ __launch_bounds__(kThreads) void k_onesweep(int n, const uint32_t *__
         __hip_atomic_store(st, kFlagIncl | (prefix + mine), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 
     // digit base: keys of smaller digits (all tiles) + this digit in earlier tiles
-    const uint32_t g = ghist[t];
+    uint32_t g = 0;
+    for (int c = 0; c < kHistCopies; c++) g += ghist[(size_t)c * 4 * 256 + t];
     scan[t] = g;
     __syncthreads();
     for (int off = 1; off < 256; off <<= 1) {
@@ -133,11 +134,13 @@ int sort_passes(int bits)
     return passes < 1 ? 1 : (passes > 4 ? 4 : passes);
 }
 
+size_t sort_hist_words() { return (size_t)kHistCopies * 4 * 256; }
+
 size_t sort_scratch_words(int n, int max_passes)
 {
     const size_t tiles = (size_t)((n + kTile - 1) / kTile) + 1;
-    // ghist [4][256] + tile counters [4] + status [passes][tiles][256]
-    return 4 * 256 + 4 + (size_t)max_passes * tiles * 256;
+    // ghist [copies][4][256] + tile counters [4] + status [passes][tiles][256]
+    return (size_t)kHistCopies * 4 * 256 + 4 + (size_t)max_passes * tiles * 256;
 }
 
 int radix_sort(hipStream_t st, SortBuffers &b, int bits, int parity)
@@ -148,7 +151,7 @@ int radix_sort(hipStream_t st, SortBuffers &b, int bits, int parity)
     const int tiles = (n + kTile - 1) / kTile;
     b.tiles = tiles;
     uint32_t *ghist = b.scratch[parity];
-    uint32_t *tilectr = ghist + 4 * 256;
+    uint32_t *tilectr = ghist + (size_t)kHistCopies * 4 * 256;
     uint32_t *status = tilectr + 4;
     uint32_t *ki = b.k0, *vi = b.v0, *ko = b.k1, *vo = b.v1;
     for (int p = 0; p < passes; p++) {

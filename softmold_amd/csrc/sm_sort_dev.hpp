@@ -39,12 +39,19 @@ __device__ inline void hist_add(uint32_t (*h)[256], uint32_t key, bool valid, in
     }
 }
 
-// Flushes a block histogram into the global digit histograms ghist[passes][256].
+// The global digit histograms are kept in kHistCopies copies, ghist[copy][4][256], and
+// block b adds into copy b % kHistCopies, so the atomics on a busy digit (the high digits
+// of a compact system are shared by every block) spread over that many addresses; the
+// sort sums the copies.
+constexpr int kHistCopies = 16;
+
+// Flushes a block histogram into its copy of the global digit histograms.
 // Needs blockDim.x == 256 and a __syncthreads() before the call.
 __device__ inline void hist_flush(uint32_t (*h)[256], uint32_t *ghist, int passes)
 {
+    uint32_t *g = ghist + (size_t)(blockIdx.x % kHistCopies) * 4 * 256;
     for (int p = 0; p < passes; p++)
-        if (h[p][threadIdx.x]) atomicAdd(&ghist[p * 256 + threadIdx.x], h[p][threadIdx.x]);
+        if (h[p][threadIdx.x]) atomicAdd(&g[p * 256 + threadIdx.x], h[p][threadIdx.x]);
 }
 
 }  // namespace smdev
