@@ -50,6 +50,19 @@ typedef struct sm_system {
     double cutoff, dt, gamma, temperature;
     double delta_lxy, tension;  /* MC box move every 8 steps when delta_lxy != 0 (MD.cpp:589) */
     uint64_t seed;              /* counter-RNG key: noise = Philox4x32-10(seed, step, particle) */
+    /* Generic BOND and BEND records (MDcuda.cu:162-184; forces system.h:1881-2035), each
+     * with its own two constants; n_bonds / n_bends may be 0 with NULL arrays. */
+    int32_t n_bonds;
+    const int32_t *bond;        /* 2 per bond: particle indices */
+    const double *bond_const;   /* 2 per bond: bond length, k_bond (harmonicF, MD.h:384-403) */
+    int32_t n_bends;
+    const int32_t *bend;        /* 3 per bend */
+    const double *bend_const;   /* 2 per bend: cos-theta term, k_bend (bendF, MD.h:683-723) */
+    /* > 0: the Langevin sigma = sqrt(6 T gamma / dt) is taken once at this temperature and
+     * kept when the temperature changes, as the gammaType thermostat of MD.cpp:135-138 does
+     * (langevin.h:235-241 computes its per-type sigma on the first call only); 0: sigma
+     * follows the temperature (scalar gamma, MD.cpp:131-134). */
+    double sigma_temperature;
 } sm_system;
 
 const char *sm_last_error(void);
@@ -102,6 +115,17 @@ int sm_mc_hist(sm_ctx *ctx, int32_t replica, int32_t cap, double *accepted, doub
 /* {U_pair, U_chain, K} of each replica in FP64 at the current state
  * (dataExtraction.h:840-842,942-948); out holds 3 per replica. */
 int sm_measure(sm_ctx *ctx, double *out);
+/* Mean distance of each replica's beads of type head_type (2 = HEAD, MD.h:21) from their
+ * centre of mass, computed on the device in FP64 from the resident positions: the
+ * per-frame computation of analysis/vesicleRadius.cpp:71-175, with its image rules (the
+ * centre-of-mass loop images heads towards the replica's first bead; the radius loop adds
+ * its image term).  radius holds one value per replica (NaN for a replica without heads). */
+int sm_vesicle_radius(sm_ctx *ctx, int32_t head_type, double *radius);
+/* Statistics of the generic records at the current state, 4 per replica, as MD writes them
+ * to lBond_name.dat and bend_name.dat (dataExtraction.h:861-930,1009-1027): mean bond
+ * length (0/0 = NaN without BOND records), then the bends' mean cos theta and mean arm
+ * lengths (sums, not means, when there is at most one BEND record). */
+int sm_bond_stats(sm_ctx *ctx, double *out);
 /* Conservative step-0 forces with FP32 pair arithmetic: pair part (CellOpt::computeForce,
  * cellOpt.h:712-926) and CHAIN part (Blob::doChainForce, system.h:1782-1866), n*3 each;
  * either pointer may be NULL. */
@@ -166,7 +190,9 @@ int sm_mpd_free(sm_mpd *m);
 /* Borrowed view of the system (pointers valid until the handle changes or is freed). */
 int sm_mpd_system(const sm_mpd *m, sm_system *out);
 int sm_mpd_get(const sm_mpd *m, const char *key, double *value);   /* scalar keys */
-int sm_mpd_set(sm_mpd *m, const char *key, double value);          /* scalar keys; marks present */
+/* Scalar keys; marks the key present, except tension and deltaLXY at 0 and deltaT <= 0,
+ * which leave the file as the Blob setters do (system.h:4461-4546). */
+int sm_mpd_set(sm_mpd *m, const char *key, double value);
 int sm_mpd_set_state(sm_mpd *m, const double *pos, const double *vel, const double box[3]);
 /* generate/liposome.cpp:1-97 with models/lipidModels.h:1313-1469. */
 int sm_gen_liposome(int32_t seed, int32_t n_lipids, double areal_density, sm_mpd **out);

@@ -624,9 +624,67 @@ static double chain_terms(const orc_sys *s, int mode, double *acc, const double 
     return total;
 }
 
-void orc_chain_force(const orc_sys *s, double *acc) { chain_terms(s, 0, acc, NULL); }
-double orc_chain_potential(const orc_sys *s) { return chain_terms(s, 1, NULL, NULL); }
-double orc_chain_dpotential(const orc_sys *s, const double scale[3]) { return chain_terms(s, 2, NULL, scale); }
+/* Generic BOND and BEND records, each with its own two constants: doBondForce and
+ * doBendForce (system.h:1881-2035), doBondPotential / doBendPotential (:2718-2780,
+ * :2782-2830), doBondDPotential / doBendDPotential (:3399-3432, :3477-3530; the imaged
+ * differences are scaled, not the positions).  The reference visits molecules in file
+ * order; here CHAIN blocks come first, then BOND records, then BEND records, which only
+ * changes the FP64 summation order of a bead's bonded force. */
+static double generic_terms(const orc_sys *s, int mode, double *acc, const double *S)
+{
+    double total = 0.0;
+    for (int j = 0; j < s->nbond; j++) {
+        const int32_t f = s->bond[2 * j], g = s->bond[2 * j + 1];
+        const double *c = s->bond_const + 2 * j;
+        double d[3];
+        min_image_diff(s->pos + 3 * (size_t)f, s->pos + 3 * (size_t)g, s->box, d);
+        if (mode == 0) {
+            harmonic_f(d, acc + 3 * (size_t)f, acc + 3 * (size_t)g, c);
+        } else if (mode == 1) {
+            total += harmonic_p(d, c);
+        } else {
+            const double oldP = harmonic_p(d, c);
+            for (int k = 0; k < 3; k++) d[k] *= S[k];
+            const double newP = harmonic_p(d, c);
+            total += (oldP - newP);
+        }
+    }
+    for (int j = 0; j < s->nbend; j++) {
+        const int32_t f = s->bend[3 * j], g = s->bend[3 * j + 1], h = s->bend[3 * j + 2];
+        const double *c = s->bend_const + 2 * j;
+        double da[3], db[3];
+        min_image_diff(s->pos + 3 * (size_t)f, s->pos + 3 * (size_t)g, s->box, da);
+        min_image_diff(s->pos + 3 * (size_t)g, s->pos + 3 * (size_t)h, s->box, db);
+        if (mode == 0) {
+            bend_f(da, db, acc + 3 * (size_t)f, acc + 3 * (size_t)g, acc + 3 * (size_t)h, c);
+        } else if (mode == 1) {
+            total += bend_p(da, db, c);
+        } else {
+            const double oldP = bend_p(da, db, c);
+            for (int k = 0; k < 3; k++) { da[k] *= S[k]; db[k] *= S[k]; }
+            const double newP = bend_p(da, db, c);
+            total += (oldP - newP);
+        }
+    }
+    return total;
+}
+
+/* all bonded terms: CHAIN blocks, then the generic records */
+void orc_chain_force(const orc_sys *s, double *acc)
+{
+    chain_terms(s, 0, acc, NULL);
+    generic_terms(s, 0, acc, NULL);
+}
+double orc_chain_potential(const orc_sys *s)
+{
+    double u = chain_terms(s, 1, NULL, NULL);
+    return u + generic_terms(s, 1, NULL, NULL);
+}
+double orc_chain_dpotential(const orc_sys *s, const double scale[3])
+{
+    double u = chain_terms(s, 2, NULL, scale);
+    return u + generic_terms(s, 2, NULL, scale);
+}
 
 /* ------------------------------------------------------------------------------------
  * Thermostat and integrator.
@@ -852,4 +910,43 @@ void orc_md_measure(orc_md *m, double *U, double *K)
     u += orc_chain_potential(m->s);
     *U = u;
     *K = orc_kinetic(m->s);
+}
+
+/* ------------------------------------------------------------------------------------
+ * Analysis.
+ * ---------------------------------------------------------------------------------- */
+static double vr_image(double d, double L)
+{
+    double m = 0.0;
+    if (d > L / 2.0) m = L;
+    if (d < -L / 2.0) m = -L;
+    return m;
+}
+
+/* One frame of analysis/vesicleRadius.cpp:71-165: the centre of mass of the beads of type
+ * `head`, each moved by minImg towards particle 0 (subtracted, :88-110), then the mean
+ * distance from it with the image term ADDED to the difference (:137-151). */
+double orc_vesicle_radius(const double *pos, const int32_t *type, int32_t n, const double box[3], int32_t head)
+{
+    double com[3] = {0.0, 0.0, 0.0}, nh = 0.0;
+    for (int i = 0; i < n; i++) {
+        if (type[i] != head) continue;
+        for (int k = 0; k < 3; k++) {
+            const double d = pos[3 * (size_t)i + k] - pos[k];
+            com[k] += pos[3 * (size_t)i + k] - vr_image(d, box[k]);
+        }
+        nh++;
+    }
+    for (int k = 0; k < 3; k++) com[k] /= nh;
+    double radius = 0.0;
+    for (int i = 0; i < n; i++) {
+        if (type[i] != head) continue;
+        double d[3];
+        for (int k = 0; k < 3; k++) {
+            d[k] = pos[3 * (size_t)i + k] - com[k];
+            d[k] += vr_image(d[k], box[k]);
+        }
+        radius += sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    }
+    return radius / nh;
 }

@@ -31,6 +31,12 @@ typedef struct {
     double box[3];          /* in/out (MC move) */
     double cutoff, dt, gamma, temp, delta_lxy, tension;
     uint32_t seed;
+    int32_t nbond;              /* generic BOND records: pairs ... */
+    const int32_t *bond;        /* nbond*2 */
+    const double *bond_const;   /* nbond*2: bond length, k_bond (harmonicF, MD.h:384-403) */
+    int32_t nbend;              /* generic BEND records: triples ... */
+    const int32_t *bend;        /* nbend*3 */
+    const double *bend_const;   /* nbend*2: cos-theta term, k_bend (bendF, MD.h:683-723) */
 } orc_sys;
 
 /* OpenMP thread count for the oracle's parallel loops (1 gives the reference's
@@ -86,7 +92,9 @@ void orc_md_destroy(orc_md *m);
 void orc_md_init_forces(orc_md *m);            /* MD.cpp:186-262 */
 int32_t orc_md_run(orc_md *m, int32_t start, int32_t end); /* MD.cpp:335-729; returns #accepted MC moves */
 int32_t orc_md_trials(const orc_md *m);
-double orc_md_objective(orc_md *m, int32_t nsteps);         /* mean U after each kick, steps 1..nsteps */
+double orc_md_objective(orc_md *m, int32_t nsteps);
+/* analysis/vesicleRadius.cpp:71-175, one frame */
+double orc_vesicle_radius(const double *pos, const int32_t *type, int32_t n, const double box[3], int32_t head);         /* mean U after each kick, steps 1..nsteps */
 void orc_md_measure(orc_md *m, double *U, double *K);      /* dataExtraction.h:827-1050 */
 
 #ifdef __cplusplus

@@ -17,7 +17,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-CHAIN = 8  # molecule type id (include/algorithms/molecules.h:16)
+BOND, BEND, CHAIN = 6, 7, 8  # molecule type ids (include/algorithms/molecules.h:14-16)
 
 
 def build() -> None:
@@ -49,6 +49,8 @@ class OrcSys(C.Structure):
         ("cutoff", C.c_double), ("dt", C.c_double), ("gamma", C.c_double),
         ("temp", C.c_double), ("delta_lxy", C.c_double), ("tension", C.c_double),
         ("seed", C.c_uint32),
+        ("nbond", C.c_int32), ("bond", C.POINTER(C.c_int32)), ("bond_const", C.POINTER(C.c_double)),
+        ("nbend", C.c_int32), ("bend", C.POINTER(C.c_int32)), ("bend_const", C.POINTER(C.c_double)),
     ]
 
 
@@ -82,6 +84,7 @@ def _declare(L):
         "orc_verlet_first": (None, [P(OrcSys)]),
         "orc_verlet_second": (None, [P(OrcSys)]),
         "orc_kinetic": (d, [P(OrcSys)]),
+        "orc_vesicle_radius": (d, [P(d), P(i32), i32, P(d), i32]),
         "orc_md_create": (C.c_void_p, [P(OrcSys), i32, i32]),
         "orc_md_destroy": (None, [C.c_void_p]),
         "orc_md_init_forces": (None, [C.c_void_p]),
@@ -113,7 +116,7 @@ _NPART = {6: 2, 7: 3, 8: 3}     # indices per record
 
 def read_mpd(path: str) -> dict:
     """Parse the keys a lipid MD system uses.  Molecule records of types other than
-    BOND/BEND/CHAIN are rejected (the oracle covers CHAIN only)."""
+    BOND/BEND/CHAIN are rejected."""
     with open(path) as f:
         tok = f.read().split()
     out: dict = {"present": []}
@@ -179,7 +182,7 @@ class System:
 
     def __init__(self, pos, type_, vel, box, fconst, uconst, ntypes, chains, chain_const,
                  cutoff=2.0, dt=0.02, gamma=1.0, temp=3.0, delta_lxy=0.0, tension=0.0,
-                 seed=0):
+                 seed=0, bonds=None, bond_const=None, bends=None, bend_const=None):
         self.pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3).copy()
         self.type = np.ascontiguousarray(type_, dtype=np.int32).copy()
         self.vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3).copy()
@@ -188,6 +191,14 @@ class System:
         self.uconst = np.ascontiguousarray(uconst, dtype=np.float64).copy()
         self.chain = np.ascontiguousarray(chains, dtype=np.int32).reshape(-1, 3).copy()
         self.chain_const = np.ascontiguousarray(chain_const, dtype=np.float64).reshape(-1, 4).copy()
+        # generic BOND / BEND records, two constants per record
+        self.bond = np.ascontiguousarray(bonds if bonds is not None else [], dtype=np.int32).reshape(-1, 2).copy()
+        self.bond_const = np.ascontiguousarray(bond_const if bond_const is not None else [],
+                                               dtype=np.float64).reshape(-1, 2).copy()
+        self.bend = np.ascontiguousarray(bends if bends is not None else [], dtype=np.int32).reshape(-1, 3).copy()
+        self.bend_const = np.ascontiguousarray(bend_const if bend_const is not None else [],
+                                               dtype=np.float64).reshape(-1, 2).copy()
+        assert len(self.bond_const) == len(self.bond) and len(self.bend_const) == len(self.bend)
         self.s = OrcSys()
         s = self.s
         s.n = len(self.type)
@@ -199,23 +210,32 @@ class System:
         s.box[0], s.box[1], s.box[2] = box
         s.cutoff, s.dt, s.gamma, s.temp = cutoff, dt, gamma, temp
         s.delta_lxy, s.tension, s.seed = delta_lxy, tension, seed & 0xFFFFFFFF
+        s.nbond, s.bond, s.bond_const = len(self.bond), _ip(self.bond), _dp(self.bond_const)
+        s.nbend, s.bend, s.bend_const = len(self.bend), _ip(self.bend), _dp(self.bend_const)
 
     @classmethod
     def from_mpd(cls, path: str) -> "System":
         m = read_mpd(path)
         chains, cc = [], []
+        rec = {BOND: ([], []), BEND: ([], [])}
         for mol in m.get("molecule", []):
-            if mol["type"] != CHAIN:
-                raise ValueError("oracle System supports CHAIN molecules only")
-            for rec in mol["bonds"]:
-                chains.append(rec)
-                cc.append(mol["const"])
+            if mol["type"] == CHAIN:
+                for r in mol["bonds"]:
+                    chains.append(r)
+                    cc.append(mol["const"])
+            elif mol["type"] in rec:
+                for r in mol["bonds"]:
+                    rec[mol["type"]][0].append(r)
+                    rec[mol["type"]][1].append(mol["const"])
+            else:
+                raise ValueError("oracle System supports CHAIN, BOND and BEND molecules only")
         return cls(m["pos"], m["type"], m["vel"], m["size"], m["twoBodyFconst"],
                    m["twoBodyUconst"], m["nTypes"], np.array(chains).reshape(-1, 3),
                    np.array(cc).reshape(-1, 4), cutoff=m["cutoff"], dt=m["deltaT"],
                    gamma=m.get("gamma", 0.0), temp=m["initialTemp"],
                    delta_lxy=m.get("deltaLXY", 0.0), tension=m.get("tension", 0.0),
-                   seed=m["seed"])
+                   seed=m["seed"], bonds=rec[BOND][0], bond_const=rec[BOND][1],
+                   bends=rec[BEND][0], bend_const=rec[BEND][1])
 
     @property
     def box(self):
@@ -310,6 +330,42 @@ class MD:
     def objective(self, nsteps: int) -> float:
         """Init forces, steps 1..nsteps; mean U_pair + U_chain after each kick."""
         return lib().orc_md_objective(self.h, nsteps)
+
+
+def vesicle_radius(pos, type_, box, head: int = 2) -> float:
+    """One frame of analysis/vesicleRadius.cpp:71-175 (orc_vesicle_radius)."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    type_ = np.ascontiguousarray(type_, dtype=np.int32)
+    box = np.ascontiguousarray(box, dtype=np.float64)
+    return lib().orc_vesicle_radius(_dp(pos), _ip(type_), len(type_), _dp(box), int(head))
+
+
+def bond_stats(pos, box, bonds, bends):
+    """dataExtraction::compute's bond and bend statistics (dataExtraction.h:861-930,
+    1009-1027): mean bond length (0/0 without bonds), then the bends' cos theta and arm
+    lengths, divided by the bend count only when it exceeds 1.  Differences are imaged
+    with >= / <= against half the box."""
+    pos = np.asarray(pos, np.float64)
+    box = np.asarray(box, np.float64)
+
+    def img(d):
+        d = d.copy()
+        for k in range(3):
+            d[..., k] = np.where(d[..., k] >= box[k] / 2.0, d[..., k] - box[k], d[..., k])
+            d[..., k] = np.where(d[..., k] <= -box[k] / 2.0, d[..., k] + box[k], d[..., k])
+        return d
+
+    bonds = np.asarray(bonds, np.int64).reshape(-1, 2)
+    bends = np.asarray(bends, np.int64).reshape(-1, 3)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        lb = np.float64(np.linalg.norm(img(pos[bonds[:, 0]] - pos[bonds[:, 1]]), axis=1).sum()) / np.float64(len(bonds))
+    da = img(pos[bends[:, 0]] - pos[bends[:, 1]])
+    db = img(pos[bends[:, 1]] - pos[bends[:, 2]])
+    ra, rb = np.linalg.norm(da, axis=1), np.linalg.norm(db, axis=1)
+    out = np.array([lb, ((da * db).sum(1) / (ra * rb)).sum(), ra.sum(), rb.sum()])
+    if len(bends) > 1:
+        out[1:] /= len(bends)
+    return out
 
 
 def set_threads(n: int):

@@ -43,6 +43,9 @@ class SmSystem(C.Structure):
         ("cutoff", C.c_double), ("dt", C.c_double), ("gamma", C.c_double),
         ("temperature", C.c_double), ("delta_lxy", C.c_double), ("tension", C.c_double),
         ("seed", C.c_uint64),
+        ("n_bonds", C.c_int32), ("bond", C.POINTER(C.c_int32)), ("bond_const", C.POINTER(C.c_double)),
+        ("n_bends", C.c_int32), ("bend", C.POINTER(C.c_int32)), ("bend_const", C.POINTER(C.c_double)),
+        ("sigma_temperature", C.c_double),
     ]
 
 
@@ -51,7 +54,7 @@ EXPORTS = [
     "sm_last_error", "sm_version", "sm_create", "sm_create_batch", "sm_destroy", "sm_replicas",
     "sm_sync", "sm_download_state", "sm_upload_state", "sm_set_temperature", "sm_init_forces",
     "sm_second_half", "sm_step_first", "sm_step_finish", "sm_run", "sm_mc_stats", "sm_mc_hist",
-    "sm_measure", "sm_compute_forces", "sm_cell_keys", "sm_pair_count", "sm_pair_list",
+    "sm_measure", "sm_vesicle_radius", "sm_bond_stats", "sm_compute_forces", "sm_cell_keys", "sm_pair_count", "sm_pair_list",
     "sm_potential_delta", "sm_tune", "sm_tune_get", "sm_autotune", "sm_set_params", "sm_get_params",
     "sm_adjoint", "sm_profile_enable",
     "sm_profile_read", "sm_mpd_read", "sm_mpd_write", "sm_mpd_free", "sm_mpd_system",
@@ -96,6 +99,8 @@ def _declare(L):
         "sm_mc_stats": (C.c_int, [vp, P(i64), P(i64)]),
         "sm_mc_hist": (C.c_int, [vp, i32, i32, P(d), P(d), P(i32)]),
         "sm_measure": (C.c_int, [vp, P(d)]),
+        "sm_vesicle_radius": (C.c_int, [vp, i32, P(d)]),
+        "sm_bond_stats": (C.c_int, [vp, P(d)]),
         "sm_compute_forces": (C.c_int, [vp, P(d), P(d)]),
         "sm_cell_keys": (C.c_int, [vp, P(i32)]),
         "sm_pair_count": (C.c_int, [vp, P(i32), P(i64)]),
@@ -197,6 +202,13 @@ class Mpd:
             "n_types": nt, "box": np.array(list(s.box)), "cutoff": s.cutoff, "dt": s.dt,
             "gamma": s.gamma, "temperature": s.temperature, "delta_lxy": s.delta_lxy,
             "tension": s.tension, "seed": int(s.seed),
+            "bond": arr(s.bond, 2 * s.n_bonds, np.int32).reshape(-1, 2) if s.n_bonds else np.zeros((0, 2), np.int32),
+            "bond_const": (arr(s.bond_const, 2 * s.n_bonds, np.float64).reshape(-1, 2) if s.n_bonds
+                           else np.zeros((0, 2))),
+            "bend": arr(s.bend, 3 * s.n_bends, np.int32).reshape(-1, 3) if s.n_bends else np.zeros((0, 3), np.int32),
+            "bend_const": (arr(s.bend_const, 2 * s.n_bends, np.float64).reshape(-1, 2) if s.n_bends
+                           else np.zeros((0, 2))),
+            "sigma_temperature": s.sigma_temperature,
         }
 
     def set_state(self, pos, vel, box):
@@ -230,6 +242,11 @@ def _pack(d: dict, keep: list) -> SmSystem:
         "uconst": np.ascontiguousarray(d["uconst"], np.float64),
         "chain": np.ascontiguousarray(d["chain"], np.int32).reshape(-1, 3),
         "chain_const": np.ascontiguousarray(d["chain_const"], np.float64).reshape(-1, 4),
+        # generic BOND / BEND records (optional keys)
+        "bond": np.ascontiguousarray(d.get("bond", np.zeros((0, 2))), np.int32).reshape(-1, 2),
+        "bond_const": np.ascontiguousarray(d.get("bond_const", np.zeros((0, 2))), np.float64).reshape(-1, 2),
+        "bend": np.ascontiguousarray(d.get("bend", np.zeros((0, 3))), np.int32).reshape(-1, 3),
+        "bend_const": np.ascontiguousarray(d.get("bend_const", np.zeros((0, 2))), np.float64).reshape(-1, 2),
     }
     if d.get("acc") is not None:
         k["acc"] = np.ascontiguousarray(d["acc"], np.float64).reshape(-1, 3)
@@ -246,6 +263,9 @@ def _pack(d: dict, keep: list) -> SmSystem:
     s.cutoff, s.dt, s.gamma, s.temperature = d["cutoff"], d["dt"], d["gamma"], d["temperature"]
     s.delta_lxy, s.tension = d["delta_lxy"], d["tension"]
     s.seed = int(d["seed"]) & 0xFFFFFFFFFFFFFFFF
+    s.n_bonds, s.bond, s.bond_const = len(k["bond"]), _ip(k["bond"]), _dp(k["bond_const"])
+    s.n_bends, s.bend, s.bend_const = len(k["bend"]), _ip(k["bend"]), _dp(k["bend_const"])
+    s.sigma_temperature = float(d.get("sigma_temperature", 0.0))
     return s
 
 
@@ -370,6 +390,20 @@ class Solver:
         """(U_pair, U_chain, K) per replica, FP64."""
         out = np.zeros((self.nrep, 3))
         _check(lib().sm_measure(self.h, _dp(out)))
+        return self._one(out)
+
+    def vesicle_radius(self, head_type: int = 2):
+        """Mean distance of the head beads from their centre of mass, per replica, computed
+        on the device (analysis/vesicleRadius.cpp:71-175)."""
+        out = np.zeros(self.nrep)
+        _check(lib().sm_vesicle_radius(self.h, int(head_type), _dp(out)))
+        return self._one(out)
+
+    def bond_stats(self):
+        """(mean bond length, mean bend cos theta, mean bend arm lengths a, b) per replica,
+        as the reference's lBond_ and bend_ files give them (dataExtraction.h:861-930)."""
+        out = np.zeros((self.nrep, 4))
+        _check(lib().sm_bond_stats(self.h, _dp(out)))
         return self._one(out)
 
     def compute_forces(self):

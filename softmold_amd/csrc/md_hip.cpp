@@ -10,8 +10,10 @@
 //  - every storeInterval (i != start): after the first half step, writes name.tmp.mpd then
 //    name.mpd and appends a frame to frames_name.xyz (MD.cpp:373-381, MDcuda.cu:351-365);
 //  - every measureInterval (i != start): appends `time\tvalue` to potential_name.dat and
-//    kinetic_name.dat, `time\tLx\tLy\tLz` to size_name.dat, `time\tT` to temp_name.dat
-//    (dataExtraction.h:992-1050, dataCollection.h:677-690) and `time\tseconds` on stderr;
+//    kinetic_name.dat, `time\tLx\tLy\tLz` to size_name.dat, the mean bond length to
+//    lBond_name.dat, `time\tcos\tla\tlb` of the bends to bend_name.dat, `time\tT` to
+//    temp_name.dat (dataExtraction.h:992-1050, dataCollection.h:677-690) and
+//    `time\tseconds` on stderr;
 //  - temperature ramp by tempStepInterval (MD.cpp:320-325,369-371);
 //  - MC box move every 8 steps when deltaLXY != 0 (MD.cpp:589-721), resizeHist_name.dat
 //    and "Resize acceptance ratio" at exit (MD.cpp:731-748).
@@ -78,11 +80,18 @@ int run(const char *name)
     const std::string nm(name);
     const std::string fFrames = "frames_" + nm + ".xyz", fPot = "potential_" + nm + ".dat",
                       fKin = "kinetic_" + nm + ".dat", fSize = "size_" + nm + ".dat", fTemp = "temp_" + nm + ".dat",
-                      fHist = "resizeHist_" + nm + ".dat";
+                      fHist = "resizeHist_" + nm + ".dat", fBond = "lBond_" + nm + ".dat",
+                      fBend = "bend_" + nm + ".dat";
     int rc = 0;
     sm_ctx *ctx = nullptr;
     try {
         sm_system sys;
+        if (sm_mpd_system(mpd, &sys) != SM_OK && std::string(sm_last_error()) == "No gamma available!") {
+            // MD.cpp:139-143: neither gamma nor gammaType; message on stdout, exit status 0
+            std::cout << "Error(main): No gamma available!\n";
+            sm_mpd_free(mpd);
+            return 0;
+        }
         check(sm_mpd_system(mpd, &sys), "system");
         const int n = sys.n_particles;
         const double dt = sys.dt;
@@ -111,11 +120,14 @@ int run(const char *name)
         check(sm_create(&sys, &ctx), "create");
         check(sm_init_forces(ctx), "initial forces");
         if (t0 == 0) {
-            double m[3];
+            double m[3], bs[4];
             check(sm_measure(ctx, m), "measure");
+            check(sm_bond_stats(ctx, bs), "bond statistics");
             const double u = m[0] + m[1];
             append_line(fPot, t0, &u, 1);
             append_line(fSize, t0, box, 3);
+            append_line(fBond, t0, bs, 1);
+            append_line(fBend, t0, bs + 1, 3);
             append_line(fTemp, t0, &temp, 1);
             append_line(fKin, t0, &m[2], 1);
             append_frame(fFrames, n, type.data(), sys.pos);
@@ -157,8 +169,11 @@ int run(const char *name)
             const bool measure = i % measureint == 0 && i != startInt;
             // the box is measured before this step's MC move (MD.cpp:513-589) and no
             // earlier part of the step changes it
-            double m[3], b[3];
-            if (measure) check(sm_download_state(ctx, nullptr, nullptr, nullptr, b), "box");
+            double m[3], b[3], bs[4];
+            if (measure) {
+                check(sm_download_state(ctx, nullptr, nullptr, nullptr, b), "box");
+                check(sm_bond_stats(ctx, bs), "bond statistics");   // positions as measured
+            }
             check(sm_step_finish(ctx, i, measure ? m : nullptr), "step");
             if (measure) {
                 const time_t last = current;
@@ -167,6 +182,8 @@ int run(const char *name)
                 const double u = m[0] + m[1];
                 append_line(fPot, t, &u, 1);
                 append_line(fSize, t, b, 3);
+                append_line(fBond, t, bs, 1);
+                append_line(fBend, t, bs + 1, 3);
                 append_line(fTemp, t, &temp, 1);
                 append_line(fKin, t, &m[2], 1);
                 current = time(nullptr);

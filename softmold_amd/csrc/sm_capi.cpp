@@ -58,6 +58,7 @@ constexpr long long kStepAdvance = -1, kStepKeep = -2;
 // host mirror of a replica's layout
 struct RepHost {
     int first = 0, count = 0, cfirst = 0, ccount = 0;
+    int nbond = 0, nbend = 0;   // generic BOND / BEND records
     long long capacity = 0, cellbase = 0;
 };
 
@@ -107,6 +108,11 @@ struct sm_ctx {
     double *uconstD = nullptr;
     int2 *chainPerBead = nullptr, *chainList = nullptr;
     double4 *cconstD = nullptr;
+    // generic BOND / BEND records, per bead (CSR); gptr == nullptr when there are none
+    int *gptr = nullptr;
+    GenEnt *gent = nullptr;
+    double2 *gconstD = nullptr;
+    GenTables gen() const { return GenTables{gptr, gent, gconstD}; }
     // host copies of the constants (the adjoint changes them)
     std::vector<double> hfconst, huconst;
     std::vector<double4> hcconst;
@@ -146,6 +152,7 @@ struct sm_ctx {
         dfree(beads); dfree(sbeads); dfree(vel); dfree(acc); dfree(fpair); dfree(fscratch);
         dfree(k0); dfree(v0); dfree(k1); dfree(v1); dfree(hist[0]); dfree(hist[1]); dfree(cstart); dfree(cend);
         dfree(reps); dfree(glob); dfree(fconstF); dfree(uconstD); dfree(chainPerBead); dfree(chainList);
+        dfree(gptr); dfree(gent); dfree(gconstD);
         dfree(cconstD); dfree(ebead); dfree(echain); dfree(dres); dfree(dcounts); dfree(dpairs); dfree(dnp);
         dfree(adjF); dfree(adjDF); dfree(adjDU);
         dfree(segB.tile); dfree(segB.rep_first); dfree(segB.tilesum);
@@ -323,7 +330,8 @@ struct sm_ctx {
     void bonded(int mode, float4 *fchain_out)
     {
         timed(K_BONDED, [&] {
-            launch_bonded_kick(st, P, beads, chainPerBead, cconstD, fpair, vel, acc, reps, glob, mode, fchain_out);
+            launch_bonded_kick(st, P, beads, chainPerBead, cconstD, gen(), fpair, vel, acc, reps, glob, mode,
+                               fchain_out);
         });
     }
     // {U_pair, U_chain, K} of every replica into dres[3r + 0..2] (device)
@@ -337,6 +345,10 @@ struct sm_ctx {
         timed(K_CHAIN_E, [&] {
             launch_chain_energy(st, nchains, chainList, cconstD, beads, reps, 0, echain);
             launch_seg_sum(st, segC, echain, meas_dev() + 1, 3);
+            if (gptr) {
+                launch_gen_energy(st, n, gen(), beads, reps, 0, ebead);
+                launch_seg_sum(st, segB, ebead, meas_dev() + 1, 3, true);
+            }
         });
         timed(K_KIN, [&] {
             launch_kinetic(st, n, vel, ebead);
@@ -354,6 +366,10 @@ struct sm_ctx {
         timed(K_CHAIN_E, [&] {
             launch_chain_energy(st, nchains, chainList, cconstD, beads, reps, 1, echain);
             launch_seg_sum(st, segC, echain, du_dev() + 1, 2);
+            if (gptr) {
+                launch_gen_energy(st, n, gen(), beads, reps, 1, ebead);
+                launch_seg_sum(st, segB, ebead, du_dev() + 1, 2, true);
+            }
         });
     }
     void mc_move()
@@ -442,7 +458,8 @@ void validate(const sm_system *s)
     if (!s->fconst || !s->uconst) invalid("fconst and uconst are required");
     if (!(s->cutoff > 0)) invalid("cutoff must be positive");
     if (!(s->dt > 0)) invalid("dt must be positive");
-    if (!(s->gamma > 0)) unsupported("only a scalar gamma > 0 is implemented (gammaType is not)");
+    if (!(s->gamma > 0)) invalid("gamma must be > 0");
+    if (!(s->sigma_temperature >= 0)) invalid("sigma_temperature must be >= 0");
     for (int d = 0; d < 3; d++) {
         if (!(s->box[d] > 0)) invalid("box lengths must be positive");
         if ((int)(s->box[d] / s->cutoff) < 3) unsupported("fewer than 3 cells along an axis");
@@ -466,6 +483,18 @@ void validate(const sm_system *s)
                     unsupported("non-symmetric two-body constant tables are not supported");
     if (s->n_chain_blocks < 0) invalid("negative chain block count");
     if (s->n_chain_blocks > 0 && (!s->chain || !s->chain_const)) invalid("chain blocks without data");
+    if (s->n_bonds < 0 || s->n_bends < 0) invalid("negative BOND / BEND record count");
+    if ((s->n_bonds > 0 && (!s->bond || !s->bond_const)) || (s->n_bends > 0 && (!s->bend || !s->bend_const)))
+        invalid("BOND / BEND records without data");
+    auto in_range = [&](const int32_t *t, int m) {
+        for (int k = 0; k < m; k++)
+            if (t[k] < 0 || t[k] >= n) invalid("a BOND / BEND record names a particle out of range");
+        for (int k = 0; k < m; k++)
+            for (int l = k + 1; l < m; l++)
+                if (t[k] == t[l]) invalid(std::string("a ") + (m == 2 ? "BOND" : "BEND") + " record repeats a particle");
+    };
+    for (int j = 0; j < s->n_bonds; j++) in_range(s->bond + 2 * j, 2);
+    for (int j = 0; j < s->n_bends; j++) in_range(s->bend + 3 * j, 3);
 }
 
 // replicas of one context share the interaction constants, cutoff and time step
@@ -538,6 +567,55 @@ sm_ctx *create(const sm_system *s, int nrep)
         nch = (int)list.size();
         c->nchains = nch;
 
+        // generic BOND / BEND records: each record listed under each of its beads
+        std::vector<int> gcount(n + 1, 0);
+        std::vector<GenEnt> grec;
+        std::vector<double2> gcst;
+        for (int r = 0; r < nrep; r++) {
+            // indices checked by validate()
+            const int off = c->rh[r].first;
+            auto bead = [&](int32_t b) { return off + b; };
+            c->rh[r].nbond = s[r].n_bonds;
+            c->rh[r].nbend = s[r].n_bends;
+            for (int j = 0; j < s[r].n_bonds; j++) {
+                grec.push_back(GenEnt{0, bead(s[r].bond[2 * j]), bead(s[r].bond[2 * j + 1]), -1});
+                gcst.push_back(make_double2(s[r].bond_const[2 * j], s[r].bond_const[2 * j + 1]));
+            }
+            for (int j = 0; j < s[r].n_bends; j++) {
+                grec.push_back(GenEnt{1, bead(s[r].bend[3 * j]), bead(s[r].bend[3 * j + 1]), bead(s[r].bend[3 * j + 2])});
+                gcst.push_back(make_double2(s[r].bend_const[2 * j], s[r].bend_const[2 * j + 1]));
+            }
+        }
+        std::vector<int> gptrH;
+        std::vector<GenEnt> gentH;
+        std::vector<double2> gcstH;
+        if (!grec.empty()) {
+            auto members = [](const GenEnt &g, int out[3]) {
+                out[0] = g.p1;
+                out[1] = g.p2;
+                out[2] = g.p3;
+                return g.kind == 0 ? 2 : 3;
+            };
+            for (const GenEnt &g : grec) {
+                int mb[3];
+                const int m = members(g, mb);
+                for (int k = 0; k < m; k++) gcount[mb[k] + 1]++;
+            }
+            gptrH.assign(n + 1, 0);
+            for (int i = 0; i < n; i++) gptrH[i + 1] = gptrH[i] + gcount[i + 1];
+            gentH.resize(gptrH[n]);
+            gcstH.resize(gptrH[n]);
+            std::vector<int> fill(gptrH.begin(), gptrH.end() - 1);
+            for (size_t e = 0; e < grec.size(); e++) {
+                int mb[3];
+                const int m = members(grec[e], mb);
+                for (int k = 0; k < m; k++) {
+                    gentH[fill[mb[k]]] = grec[e];
+                    gcstH[fill[mb[k]]++] = gcst[e];
+                }
+            }
+        }
+
         // host staging
         std::vector<Bead> hb(n);
         std::vector<float4> hv(n), ha(n);
@@ -572,7 +650,9 @@ sm_ctx *create(const sm_system *s, int nrep)
             R.seed = q.seed;
             R.temperature = q.temperature;
             R.gamma = q.gamma;
-            R.sigma = std::sqrt((6.0 * q.temperature * q.gamma) / q.dt);
+            const double Ts = q.sigma_temperature > 0 ? q.sigma_temperature : q.temperature;
+            R.sigma = std::sqrt((6.0 * Ts * q.gamma) / q.dt);
+            R.sigma_fixed = q.sigma_temperature > 0 ? 1 : 0;
             R.delta_lxy = q.delta_lxy;
             R.tension = q.tension;
             if (q.delta_lxy != 0) c->any_mc = true;
@@ -644,6 +724,15 @@ sm_ctx *create(const sm_system *s, int nrep)
         HIPCHK(hipMemcpyAsync(c->fconstF, fF.data(), sizeof(float) * fF.size(), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(c->uconstD, s[0].uconst, sizeof(double) * fF.size(), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(c->chainPerBead, perBead.data(), sizeof(int2) * n, hipMemcpyHostToDevice, st));
+        if (!gentH.empty()) {
+            c->gptr = dalloc<int>(gptrH.size());
+            c->gent = dalloc<GenEnt>(gentH.size());
+            c->gconstD = dalloc<double2>(gcstH.size());
+            HIPCHK(hipMemcpyAsync(c->gptr, gptrH.data(), sizeof(int) * gptrH.size(), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(c->gent, gentH.data(), sizeof(GenEnt) * gentH.size(), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(c->gconstD, gcstH.data(), sizeof(double2) * gcstH.size(), hipMemcpyHostToDevice,
+                                  st));
+        }
         if (!list.empty()) {
             HIPCHK(hipMemcpyAsync(c->chainList, list.data(), sizeof(int2) * list.size(), hipMemcpyHostToDevice, st));
             HIPCHK(hipMemcpyAsync(c->cconstD, ccD.data(), sizeof(double4) * ccD.size(), hipMemcpyHostToDevice, st));
@@ -812,7 +901,7 @@ int sm_set_temperature(sm_ctx *c, double T)
         std::vector<DevRep> hr = c->read_reps();
         for (auto &R : hr) {
             R.temperature = T;
-            R.sigma = std::sqrt((6.0 * T * R.gamma) / c->P.dt);
+            if (!R.sigma_fixed) R.sigma = std::sqrt((6.0 * T * R.gamma) / c->P.dt);
         }
         c->write_reps(hr);
     });
@@ -917,6 +1006,73 @@ int sm_measure(sm_ctx *c, double *out)
         HIPCHK(hipMemcpyAsync(out, c->meas_dev(), 3 * sizeof(double) * c->nrep, hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
         c->check_flags();
+    });
+}
+
+int sm_bond_stats(sm_ctx *c, double *out)
+{
+    return guarded([&] {
+        if (!c || !out) invalid("null argument");
+        const int R = c->nrep;
+        std::vector<double> h((size_t)4 * R, 0.0);
+        if (c->gptr) {
+            double *w = dalloc<double>((size_t)4 * R);
+            try {
+                for (int what = 0; what < 4; what++) {
+                    launch_gen_stats(c->st, c->n, c->gen(), c->beads, c->reps, what, c->ebead);
+                    launch_seg_sum(c->st, c->segB, c->ebead, w + what, 4);
+                }
+                c->check_launch();
+                HIPCHK(hipMemcpyAsync(h.data(), w, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->st));
+                HIPCHK(hipStreamSynchronize(c->st));
+            } catch (...) {
+                dfree(w);
+                throw;
+            }
+            dfree(w);
+        }
+        for (int r = 0; r < R; r++) {
+            // the divisions on the host, as the reference writes them: lBond / nBond even
+            // when nBond is 0; the bend sums divided only when nBend > 1
+            const int nb = c->rh[r].nbond, ne = c->rh[r].nbend;
+            double *o = out + 4 * (size_t)r;
+            const double zero = 0.0;
+            o[0] = h[4 * r] / (nb ? (double)nb : zero);
+            o[1] = h[4 * r + 1];
+            o[2] = h[4 * r + 2];
+            o[3] = h[4 * r + 3];
+            if (ne > 1)
+                for (int k = 1; k < 4; k++) o[k] /= (double)ne;
+        }
+    });
+}
+
+int sm_vesicle_radius(sm_ctx *c, int32_t head_type, double *radius)
+{
+    return guarded([&] {
+        if (!c || !radius) invalid("null argument");
+        const int R = c->nrep;
+        // [R][4] head sums, [R][3] centres, [R] distance sums
+        double *w = dalloc<double>((size_t)8 * R);
+        try {
+            double *sums = w, *com = w + 4 * R, *dist = w + 7 * R;
+            for (int what = 0; what < 4; what++) {
+                launch_head_terms(c->st, c->n, c->beads, c->reps, head_type, what, nullptr, c->ebead);
+                launch_seg_sum(c->st, c->segB, c->ebead, sums + what, 4);
+            }
+            launch_head_com(c->st, R, sums, com);
+            launch_head_terms(c->st, c->n, c->beads, c->reps, head_type, 4, com, c->ebead);
+            launch_seg_sum(c->st, c->segB, c->ebead, dist, 1);
+            c->check_launch();
+            std::vector<double> h((size_t)8 * R);
+            HIPCHK(hipMemcpyAsync(h.data(), w, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+            for (int r = 0; r < R; r++) radius[r] = h[7 * R + r] / h[4 * r + 3];
+        } catch (...) {
+            dfree(w);
+            throw;
+        }
+        dfree(w);
     });
 }
 
@@ -1289,6 +1445,7 @@ int sm_adjoint(sm_ctx *c, int32_t type_a, int32_t type_b, int32_t nsteps, int32_
     return guarded([&] {
         if (!c || !J) invalid("null argument");
         if (c->nrep != 1) unsupported("the adjoint runs on single-system contexts");
+        if (c->gptr) unsupported("the adjoint covers CHAIN molecules only (no BOND / BEND records)");
         if (nsteps < 1 || checkpoint < 1 || checkpoint > nsteps) invalid("need 1 <= checkpoint <= nsteps");
         check_pair(c, type_a, type_b);
         const int n = c->n, S = nsteps, k = checkpoint;

@@ -40,7 +40,7 @@ struct DevRep {
     int nc[3];
     int first, count;          // bead range (molecule order == sorted order)
     int cfirst, ccount;        // chain range
-    int pad0;
+    int sigma_fixed;           // 1: sigma does not follow temperature changes (gammaType)
     long long cellbase;        // first global cell key
     long long capacity;        // cells reserved for this replica
     unsigned long long seed;   // counter-RNG key
@@ -56,6 +56,14 @@ struct DevRep {
     long long trials, accepted;
     double hist_accept[kHistBins];
     double hist_reject[kHistBins];
+};
+
+// Generic BOND / BEND record as seen from one of its beads (global bead indices):
+// kind 0 = bond (p1, p2), kind 1 = bend (p1, p2, p3).  Each bead lists the records it
+// belongs to (CSR: gptr[i] .. gptr[i+1]); the record's two constants sit beside it.
+struct __align__(16) GenEnt {
+    int kind;
+    int p1, p2, p3;
 };
 
 // Context-wide device state.
@@ -89,8 +97,8 @@ struct SegTiles {
     int nrep;
     double *tilesum;   // ntiles, device
 };
-// out[r * stride] = sum of x over replica r's range
-void launch_seg_sum(hipStream_t st, const SegTiles &S, const double *x, double *out, int stride);
+// out[r * stride] = sum of x over replica r's range (add: out[r * stride] += that sum)
+void launch_seg_sum(hipStream_t st, const SegTiles &S, const double *x, double *out, int stride, bool add = false);
 
 // ---- launchers (sm_kernels.hip) ---------------------------------------------------------
 // step_set >= 0: the device step counter is set to step_set; -1: it is advanced by one
@@ -108,9 +116,15 @@ void launch_reorder(hipStream_t st, int n, const uint32_t *skey, const uint32_t 
 void launch_pair_force(hipStream_t st, const Params &P, int chunk, int variant, const Bead *sbeads,
                        const uint32_t *skey, const uint32_t *sidx, const int *cstart, const int *cend,
                        const DevRep *reps, const float *fconst, float4 *fpair);
+// gptr == NULL: CHAIN terms only; else also the generic records of each bead
+struct GenTables {
+    const int *gptr;
+    const GenEnt *gent;
+    const double2 *gconst;
+};
 void launch_bonded_kick(hipStream_t st, const Params &P, const Bead *beads, const int2 *chain,
-                        const double4 *cconst, const float4 *fpair, float4 *vel, float4 *acc, const DevRep *reps,
-                        const DevGlobal *glob, int mode, float4 *fchain_out);
+                        const double4 *cconst, const GenTables &gen, const float4 *fpair, float4 *vel, float4 *acc,
+                        const DevRep *reps, const DevGlobal *glob, int mode, float4 *fchain_out);
 enum : int { KICK_STEP = 0, KICK_INIT = 1, KICK_NONE = 2 };
 void launch_second(hipStream_t st, int n, const Bead *beads, const float4 *acc, float4 *vel, float dt);
 
@@ -127,8 +141,21 @@ void launch_pair_gather(hipStream_t st, const Params &P, int chunk, int mode, co
 // One FP64 value per chain: its energy, or U(old) - U(scaled) when delta != 0.
 void launch_chain_energy(hipStream_t st, int nchains, const int2 *chains, const double4 *cconst, const Bead *beads,
                          const DevRep *reps, int delta, double *e);
+// Generic-record energies, one FP64 value per bead: the records whose first bead it is,
+// U, or U(old) - U(scaled) when delta != 0.
+void launch_gen_energy(hipStream_t st, int n, const GenTables &gen, const Bead *beads, const DevRep *reps,
+                       int delta, double *e);
+// Generic-record statistics per bead (what 0: bond length, 1: bend cos theta, 2 and 3: the
+// bend's arm lengths), over the records whose first bead it is.
+void launch_gen_stats(hipStream_t st, int n, const GenTables &gen, const Bead *beads, const DevRep *reps, int what,
+                      double *e);
 // One FP64 kinetic energy per bead.
 void launch_kinetic(hipStream_t st, int n, const float4 *vel, double *e);
+// Vesicle-radius terms per bead (what 0..2: imaged head coordinate, 3: head count, 4: head
+// distance to com[3r..]); sums[4r..4r+3] -> com[3r..3r+2].
+void launch_head_terms(hipStream_t st, int n, const Bead *beads, const DevRep *reps, int head, int what,
+                       const double *com, double *e);
+void launch_head_com(hipStream_t st, int nrep, const double *sums, double *com);
 void launch_mc_propose(hipStream_t st, int nrep, DevRep *reps, const DevGlobal *glob);
 // du[r*2 + 0] = dU_pair, du[r*2 + 1] = dU_chain of replica r
 void launch_mc_decide(hipStream_t st, const Params &P, DevRep *reps, DevGlobal *glob, const double *du);

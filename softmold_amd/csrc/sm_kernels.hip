@@ -182,7 +182,7 @@ __device__ inline double3 bend_f(double3 da, double3 db, double c0, double kt, i
 
 __global__ __launch_bounds__(kBlock) void k_bonded_kick(int n, const Bead *__restrict__ beads,
                                                         const int2 *__restrict__ chain,
-                                                        const double4 *__restrict__ cconst,
+                                                        const double4 *__restrict__ cconst, GenTables gen,
                                                         const float4 *__restrict__ fpair, float4 *__restrict__ vel,
                                                         float4 *__restrict__ acc, const DevRep *__restrict__ reps,
                                                         const DevGlobal *__restrict__ glob, float halfdt, int mode,
@@ -222,6 +222,25 @@ __global__ __launch_bounds__(kBlock) void k_bonded_kick(int n, const Bead *__res
         }
         if (j >= 2) {  // bend (i-2, i-1, i), role 2
             const double3 f = bend_f(min_image(pm2, pm1, L0, L1, L2), min_image(pm1, p0, L0, L1, L2), K.z, K.w, 2);
+            fc.x += f.x; fc.y += f.y; fc.z += f.z;
+        }
+    }
+    if (gen.gptr) {
+        // generic BOND / BEND records (doBondForce, doBendForce, system.h:1881-2035):
+        // the same per-record force, taken from this bead's side
+        const int e1 = gen.gptr[i + 1];
+        for (int e = gen.gptr[i]; e < e1; e++) {
+            const GenEnt g = gen.gent[e];
+            const double2 K = gen.gconst[e];
+            double3 f;
+            if (g.kind == 0) {
+                f = bond_f(min_image(beads[g.p1], beads[g.p2], L0, L1, L2), K.x, K.y);
+                if (i != g.p1) { f.x = -f.x; f.y = -f.y; f.z = -f.z; }
+            } else {
+                const Bead q1 = beads[g.p1], q2 = beads[g.p2], q3 = beads[g.p3];
+                f = bend_f(min_image(q1, q2, L0, L1, L2), min_image(q2, q3, L0, L1, L2), K.x, K.y,
+                           i == g.p1 ? 0 : (i == g.p2 ? 1 : 2));
+            }
             fc.x += f.x; fc.y += f.y; fc.z += f.z;
         }
     }
@@ -297,14 +316,14 @@ __global__ __launch_bounds__(kBlock) void k_seg_tiles(const double *__restrict__
 
 __global__ __launch_bounds__(kBlock) void k_seg_final(const double *__restrict__ tilesum,
                                                       const int *__restrict__ rep_first, double *__restrict__ out,
-                                                      int stride)
+                                                      int stride, int add)
 {
     __shared__ double lds[kBlock];
     const int r = blockIdx.x;
     double s = 0.0;
     for (int t = rep_first[r] + (int)threadIdx.x; t < rep_first[r + 1]; t += kBlock) s += tilesum[t];
     const double b = block_sum(s, lds);
-    if (threadIdx.x == 0) out[(size_t)r * stride] = b;
+    if (threadIdx.x == 0) out[(size_t)r * stride] = add ? out[(size_t)r * stride] + b : b;
 }
 
 // CHAIN energies in FP64 (harmonicP MD.h:422-431, bendP MD.h:769-789, walk
@@ -370,6 +389,86 @@ __global__ __launch_bounds__(kBlock) void k_chain_energy(int nch, const int2 *__
     e[c] = accu;
 }
 
+// Generic BOND / BEND energies (doBondPotential / doBendPotential, system.h:2718-2830;
+// dPotential with the imaged differences scaled, system.h:3399-3530), one term per bead:
+// the records whose first bead it is.
+__global__ __launch_bounds__(kBlock) void k_gen_energy(int n, GenTables gen, const Bead *__restrict__ beads,
+                                                       const DevRep *__restrict__ reps, int delta,
+                                                       double *__restrict__ e)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const DevRep &R = reps[beads[i].rep];
+    const double L[3] = {R.L[0], R.L[1], R.L[2]};
+    const double S[3] = {R.scale[0], R.scale[1], R.scale[2]};
+    double accu = 0.0;
+    const int e1 = gen.gptr[i + 1];
+    for (int k = gen.gptr[i]; k < e1; k++) {
+        const GenEnt g = gen.gent[k];
+        if (g.p1 != i) continue;
+        const double2 K = gen.gconst[k];
+        double da[3], db[3], u, un = 0.0;
+        mi3(beads[g.p1], beads[g.p2], L, da);
+        if (g.kind == 0) {
+            u = harm_p(da, K.x, K.y);
+            if (delta) {
+                for (int d = 0; d < 3; d++) da[d] *= S[d];
+                un = harm_p(da, K.x, K.y);
+            }
+        } else {
+            mi3(beads[g.p2], beads[g.p3], L, db);
+            u = bend_p(da, db, K.x, K.y);
+            if (delta) {
+                for (int d = 0; d < 3; d++) { da[d] *= S[d]; db[d] *= S[d]; }
+                un = bend_p(da, db, K.x, K.y);
+            }
+        }
+        accu += delta ? (u - un) : u;
+    }
+    e[i] = accu;
+}
+
+// Bond and bend statistics of the generic records (dataExtraction.h:861-930), one term per
+// bead over the records whose first bead it is: what 0: bond length; 1: cos theta of a
+// bend; 2, 3: its two arm lengths.  These image with >= / <= against half the box, unlike
+// the forces.
+__device__ inline void mi3_closed(const Bead &p, const Bead &q, const double *L, double d[3])
+{
+    d[0] = p.x - q.x; d[1] = p.y - q.y; d[2] = p.z - q.z;
+    for (int k = 0; k < 3; k++) {
+        if (d[k] >= L[k] / 2.0) d[k] -= L[k];
+        if (d[k] <= -L[k] / 2.0) d[k] += L[k];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_gen_stats(int n, GenTables gen, const Bead *__restrict__ beads,
+                                                      const DevRep *__restrict__ reps, int what,
+                                                      double *__restrict__ e)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const DevRep &R = reps[beads[i].rep];
+    const double L[3] = {R.L[0], R.L[1], R.L[2]};
+    double accu = 0.0;
+    const int e1 = gen.gptr[i + 1];
+    for (int k = gen.gptr[i]; k < e1; k++) {
+        const GenEnt g = gen.gent[k];
+        if (g.p1 != i || (g.kind == 0) != (what == 0)) continue;
+        double da[3], db[3];
+        mi3_closed(beads[g.p1], beads[g.p2], L, da);
+        const double ra = sqrt(da[0] * da[0] + da[1] * da[1] + da[2] * da[2]);
+        if (what == 0 || what == 2) {
+            accu += ra;
+            continue;
+        }
+        mi3_closed(beads[g.p2], beads[g.p3], L, db);
+        const double rb = sqrt(db[0] * db[0] + db[1] * db[1] + db[2] * db[2]);
+        accu += what == 3 ? rb : (da[0] * db[0] + da[1] * db[1] + da[2] * db[2]) / (ra * rb);
+    }
+    e[i] = accu;
+}
+
 // Kinetic::compute (dataCollection.h:667-673), one term per bead
 __global__ __launch_bounds__(kBlock) void k_kinetic(int n, const float4 *__restrict__ vel, double *__restrict__ e)
 {
@@ -378,6 +477,64 @@ __global__ __launch_bounds__(kBlock) void k_kinetic(int n, const float4 *__restr
     const float4 v = vel[i];
     const double x = v.x, y = v.y, z = v.z;
     e[i] = (x * x + y * y + z * z) / 2.0;
+}
+
+// ---------------------------------------------------------------------------------------
+// Vesicle radius of each replica from its head beads (analysis/vesicleRadius.cpp:71-175),
+// one FP64 term per bead for the segmented sums:
+//   what 0..2: coordinate `what` of a head, moved by one box length towards the replica's
+//              first bead when farther than half a box from it (the centre-of-mass loop);
+//   what 3:    1 per head (the head count);
+//   what 4:    the head's distance to the centre com[3r..3r+2], with the second loop's
+//              image term, which the reference ADDS to the difference (d += minImg, :147-151):
+//              the two agree with a true minimum image whenever no head lies more than half
+//              a box from the centre, and differ, as the reference does, otherwise.
+// Non-head beads give 0.
+// ---------------------------------------------------------------------------------------
+__device__ inline double image_shift(double d, double L)
+{
+    double m = 0.0;
+    if (d > L / 2.0) m = L;
+    if (d < -L / 2.0) m = -L;
+    return m;
+}
+
+__global__ __launch_bounds__(kBlock) void k_head_terms(int n, const Bead *__restrict__ beads,
+                                                       const DevRep *__restrict__ reps, int head, int what,
+                                                       const double *__restrict__ com, double *__restrict__ e)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const Bead b = beads[i];
+    double v = 0.0;
+    if (b.type == head) {
+        const DevRep &R = reps[b.rep];
+        if (what < 3) {
+            const Bead p0 = beads[R.first];
+            const double x = what == 0 ? b.x : (what == 1 ? b.y : b.z);
+            const double x0 = what == 0 ? p0.x : (what == 1 ? p0.y : p0.z);
+            v = x - image_shift(x - x0, R.L[what]);
+        } else if (what == 3) {
+            v = 1.0;
+        } else {
+            const double *c = com + 3 * b.rep;
+            double dx = b.x - c[0], dy = b.y - c[1], dz = b.z - c[2];
+            dx += image_shift(dx, R.L[0]);
+            dy += image_shift(dy, R.L[1]);
+            dz += image_shift(dz, R.L[2]);
+            v = sqrt(dx * dx + dy * dy + dz * dz);
+        }
+    }
+    e[i] = v;
+}
+
+// sums[4r + 0..3] = (sum x, sum y, sum z, heads) -> com[3r + 0..2]
+__global__ void k_head_com(int nrep, const double *__restrict__ sums, double *__restrict__ com)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrep) return;
+    for (int k = 0; k < 3; k++) com[3 * r + k] = sums[4 * r + k] / sums[4 * r + 3];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -498,12 +655,12 @@ void launch_reorder(hipStream_t st, int n, const uint32_t *skey, const uint32_t 
                        clear, clear_words);
 }
 
-void launch_bonded_kick(hipStream_t st, const Params &P, const Bead *beads, const int2 *chain, const double4 *cconst,
-                        const float4 *fpair, float4 *vel, float4 *acc, const DevRep *reps, const DevGlobal *glob,
-                        int mode, float4 *fchain_out)
+void launch_bonded_kick(hipStream_t st, const Params &P, const Bead *beads, const int2 *chain,
+                        const double4 *cconst, const GenTables &gen, const float4 *fpair, float4 *vel, float4 *acc,
+                        const DevRep *reps, const DevGlobal *glob, int mode, float4 *fchain_out)
 {
-    hipLaunchKernelGGL(k_bonded_kick, dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, beads, chain, cconst, fpair, vel,
-                       acc, reps, glob, (float)(P.dt * 0.5), mode, fchain_out);
+    hipLaunchKernelGGL(k_bonded_kick, dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, beads, chain, cconst, gen, fpair,
+                       vel, acc, reps, glob, (float)(P.dt * 0.5), mode, fchain_out);
 }
 
 void launch_second(hipStream_t st, int n, const Bead *beads, const float4 *acc, float4 *vel, float halfdt)
@@ -511,11 +668,24 @@ void launch_second(hipStream_t st, int n, const Bead *beads, const float4 *acc, 
     hipLaunchKernelGGL(k_second, dim3(blocks(n)), dim3(kBlock), 0, st, n, beads, acc, vel, halfdt);
 }
 
-void launch_seg_sum(hipStream_t st, const SegTiles &S, const double *x, double *out, int stride)
+void launch_seg_sum(hipStream_t st, const SegTiles &S, const double *x, double *out, int stride, bool add)
 {
     if (S.ntiles > 0)
         hipLaunchKernelGGL(k_seg_tiles, dim3(S.ntiles), dim3(kBlock), 0, st, x, S.tile, S.tilesum);
-    hipLaunchKernelGGL(k_seg_final, dim3(S.nrep), dim3(kBlock), 0, st, S.tilesum, S.rep_first, out, stride);
+    hipLaunchKernelGGL(k_seg_final, dim3(S.nrep), dim3(kBlock), 0, st, S.tilesum, S.rep_first, out, stride,
+                       add ? 1 : 0);
+}
+
+void launch_gen_stats(hipStream_t st, int n, const GenTables &gen, const Bead *beads, const DevRep *reps, int what,
+                      double *e)
+{
+    hipLaunchKernelGGL(k_gen_stats, dim3(blocks(n)), dim3(kBlock), 0, st, n, gen, beads, reps, what, e);
+}
+
+void launch_gen_energy(hipStream_t st, int n, const GenTables &gen, const Bead *beads, const DevRep *reps,
+                       int delta, double *e)
+{
+    hipLaunchKernelGGL(k_gen_energy, dim3(blocks(n)), dim3(kBlock), 0, st, n, gen, beads, reps, delta, e);
 }
 
 void launch_chain_energy(hipStream_t st, int nchains, const int2 *chains, const double4 *cconst, const Bead *beads,
@@ -529,6 +699,17 @@ void launch_chain_energy(hipStream_t st, int nchains, const int2 *chains, const 
 void launch_kinetic(hipStream_t st, int n, const float4 *vel, double *e)
 {
     hipLaunchKernelGGL(k_kinetic, dim3(blocks(n)), dim3(kBlock), 0, st, n, vel, e);
+}
+
+void launch_head_terms(hipStream_t st, int n, const Bead *beads, const DevRep *reps, int head, int what,
+                       const double *com, double *e)
+{
+    hipLaunchKernelGGL(k_head_terms, dim3(blocks(n)), dim3(kBlock), 0, st, n, beads, reps, head, what, com, e);
+}
+
+void launch_head_com(hipStream_t st, int nrep, const double *sums, double *com)
+{
+    hipLaunchKernelGGL(k_head_com, dim3(blocks(nrep)), dim3(kBlock), 0, st, nrep, sums, com);
 }
 
 void launch_mc_propose(hipStream_t st, int nrep, DevRep *reps, const DevGlobal *glob)

@@ -13,8 +13,8 @@ using namespace smapi;
 
 struct sm_mpd {
     smio::System sys;
-    std::vector<int32_t> chain;
-    std::vector<double> chain_const;
+    std::vector<int32_t> chain, bond, bend;
+    std::vector<double> chain_const, bond_const, bend_const;
 };
 
 namespace {
@@ -111,15 +111,35 @@ int sm_mpd_system(const sm_mpd *mc, sm_system *out)
         const smio::System &s = m->sys;
         m->chain.clear();
         m->chain_const.clear();
+        m->bond.clear();
+        m->bond_const.clear();
+        m->bend.clear();
+        m->bend_const.clear();
         for (const auto &mo : s.mol) {
-            if (mo.type != smio::CHAIN)
-                unsupported("molecule type " + std::to_string(mo.type) + " is not implemented (CHAIN only)");
-            for (const auto &b : mo.bonds) {
-                m->chain.insert(m->chain.end(), {b[0], b[1], b[2]});
-                m->chain_const.insert(m->chain_const.end(), mo.constants.begin(), mo.constants.end());
+            if (mo.type == smio::CHAIN) {
+                for (const auto &b : mo.bonds) {
+                    m->chain.insert(m->chain.end(), {b[0], b[1], b[2]});
+                    m->chain_const.insert(m->chain_const.end(), mo.constants.begin(), mo.constants.end());
+                }
+            } else if (mo.type == smio::BOND || mo.type == smio::BEND) {
+                const bool bond = mo.type == smio::BOND;
+                for (const auto &b : mo.bonds) {
+                    auto &idx = bond ? m->bond : m->bend;
+                    auto &cst = bond ? m->bond_const : m->bend_const;
+                    idx.insert(idx.end(), {b[0], b[1]});
+                    if (!bond) idx.push_back(b[2]);
+                    cst.insert(cst.end(), mo.constants.begin(), mo.constants.begin() + 2);
+                }
+            } else {
+                unsupported("molecule type " + std::to_string(mo.type) +
+                            " is not implemented (CHAIN, BOND and BEND are)");
             }
         }
-        if (s.gamma <= 0) unsupported("gamma must be > 0 (per-type gammaType is not implemented)");
+        // MD.cpp:129-143: a positive gamma, else the gammaType thermostat, which applies
+        // gammaType[0] to every particle (langevin.h:254 reads type 0) with sigma fixed at
+        // the first temperature; with neither, MD stops
+        const bool typed = s.gamma <= 0 && s.present[smio::GAMMATYPE] && !s.gammaType.empty();
+        if (s.gamma <= 0 && !typed) unsupported("No gamma available!");
         std::memset(out, 0, sizeof *out);
         out->n_particles = s.nParticles;
         out->n_types = s.nTypes;
@@ -135,11 +155,18 @@ int sm_mpd_system(const sm_mpd *mc, sm_system *out)
         for (int d = 0; d < 3; d++) out->box[d] = s.size[d];
         out->cutoff = s.cutoff;
         out->dt = s.deltaT;
-        out->gamma = s.gamma;
+        out->gamma = typed ? s.gammaType[0] : s.gamma;
+        out->sigma_temperature = typed ? s.initialTemp : 0.0;
         out->temperature = s.initialTemp;
         out->delta_lxy = s.deltaLXY;
         out->tension = s.tension;
         out->seed = (uint64_t)(uint32_t)s.seed;
+        out->n_bonds = (int32_t)(m->bond.size() / 2);
+        out->bond = m->bond.data();
+        out->bond_const = m->bond_const.data();
+        out->n_bends = (int32_t)(m->bend.size() / 3);
+        out->bend = m->bend.data();
+        out->bend_const = m->bend_const.data();
     });
 }
 
@@ -178,7 +205,10 @@ int sm_mpd_set(sm_mpd *m, const char *key, double value)
             g_err = std::string(key) + " is not settable";
             return SM_ERR_INVALID;
         } else *p = value;
-        m->sys.present[k.cmd] = true;
+        // the Blob setters (system.h:4461-4546) drop these keys from the file at zero
+        if (k.cmd == smio::TENSION || k.cmd == smio::DELTALXY) m->sys.present[k.cmd] = value != 0.0;
+        else if (k.cmd == smio::DELTAT) m->sys.present[k.cmd] = value > 0.0;
+        else m->sys.present[k.cmd] = true;
         return SM_OK;
     }
     g_err = std::string("unknown scalar key ") + key;

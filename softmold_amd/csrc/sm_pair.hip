@@ -113,20 +113,39 @@ __device__ constexpr uint32_t forward_mask()
     return m;
 }
 
+// A staged candidate read as two 16-byte LDS loads, so its type arrives with its position.
+__device__ inline Cand load_cand(const Cand *p)
+{
+    const uint4 *q = reinterpret_cast<const uint4 *>(p);
+    const uint4 a = q[0], b = q[1];
+    Cand c;
+    c.x = __hiloint2double((int)a.y, (int)a.x);
+    c.y = __hiloint2double((int)a.w, (int)a.z);
+    c.z = __hiloint2double((int)b.y, (int)b.x);
+    c.t = (int)b.z;
+    c.c = (int)b.w;
+    return c;
+}
+
 // One candidate of the force kernels: the exact FP64 test r^2 < rc^2 (MD.h:798-810), then
 // the force law (MD.h:811-848) in FP32.  With r0 = r2f * rsq(r2f) (exact as an FP64
 // product), r = r0 + corr is one Newton step against the FP64 residual r^2 - r0^2, and
 // m = C - r is formed as fma(-r2f, rsq, C) - corr, so it carries no FP32 rounding of r^2
 // or r0 (near r_m the force is C1 m with C1 ~ 400).  1/r is rsq(r2f) refined by one Newton
-// step against r.
-__device__ inline void force_term(const Bead &hb, const Cand &q, bool other, double cut2, const float *Cf, float &fx,
-                                  float &fy, float &fz)
+// step against r.  C8 holds the home type's constant rows padded to 8 floats, one row
+// {r_m, C1, C2, r_c, C4, C5, 0, 0} per candidate type, read as two float4 in one LDS trip;
+// the branch r >= r_m selects (r_c, C4, C5) over (r_m, C1, C2), which is the reference's
+// row offset int(r/r_m)*3 clamped (MD.h:819).
+__device__ inline void force_term(const Bead &hb, const Cand &q, bool other, double cut2, const float4 *C8,
+                                  float &fx, float &fy, float &fz)
 {
     const double dxd = hb.x - q.x;
     const double dyd = hb.y - q.y;
     const double dzd = hb.z - q.z;
     const double r2 = sq3(dxd, dyd, dzd);
-    if (r2 < cut2 && other) {
+    // the type bound keeps the type's LDS read ahead of the branch, with the position
+    if (r2 < cut2 && other && (unsigned)q.t < (unsigned)kMaxTypes) {
+        const float4 A = C8[2 * q.t], B = C8[2 * q.t + 1];
         const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
         const float r2f = (float)r2;
         const float ri0 = __builtin_amdgcn_rsqf(r2f);
@@ -134,10 +153,10 @@ __device__ inline void force_term(const Bead &hb, const Cand &q, bool other, dou
         const float corr = (float)fma(-r0d, r0d, r2) * (0.5f * ri0);
         const float r = (float)r0d + corr;
         const float ri = ri0 * (2.0f - r * ri0);
-        const float *C = Cf + 6 * q.t;
-        const int o = (r >= C[0]) ? 3 : 0;       // int(r/r_m)*3, clamped
-        const float m = fmaf(-r2f, ri0, C[o]) - corr;
-        const float mag = ((C[o + 1] - C[o + 2] * m) * m) * ri;
+        const bool outer = r >= A.x;
+        const float c0 = outer ? A.w : A.x, c1 = outer ? B.x : A.y, c2 = outer ? B.y : A.z;
+        const float m = fmaf(-r2f, ri0, c0) - corr;
+        const float mag = ((c1 - c2 * m) * m) * ri;
         fx += dx * mag;
         fy += dy * mag;
         fz += dz * mag;
@@ -165,7 +184,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     constexpr bool kF1 = MODE == M_FORCE1;
     constexpr int kMaxC = k2P ? 128 : 256;   // staged candidates per pass
     constexpr int kStageIters = kMaxC / kWave;
-    __shared__ float sCf[kF ? 6 * kMaxTypes * kMaxTypes : 1];
+    __shared__ float4 sC8[kF ? 2 * kMaxTypes * kMaxTypes : 1];   // rows padded to 8 floats
     __shared__ double sCd[kF ? 1 : 6 * kMaxTypes * kMaxTypes];
     __shared__ Cand sCand[kWaves][kMaxC];
     __shared__ float4 sRel[k2P ? kWaves : 1][k2P ? kMaxC : 1];
@@ -176,7 +195,8 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     __shared__ int redi[kWaves][kWave];
 
     if (kF) {
-        for (int t = threadIdx.x; t < 6 * nT * nT; t += blockDim.x) sCf[t] = fconst[t];
+        float *c8 = reinterpret_cast<float *>(sC8);
+        for (int t = threadIdx.x; t < 8 * nT * nT; t += blockDim.x) c8[t] = (t & 7) < 6 ? fconst[6 * (t >> 3) + (t & 7)] : 0.f;
     } else {
         for (int t = threadIdx.x; t < 6 * nT * nT; t += blockDim.x) sCd[t] = uconst[t];
     }
@@ -218,9 +238,12 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
             // home cell centre: FP32 positions relative to it stay within 1.5 cells
             const double ox0 = (kx + 0.5) * csx, oy0 = (ky + 0.5) * csy, oz0 = (kz + 0.5) * csz;
 
-            // 1. stencil lookups, one cell per lane
+            // 1. stencil lookups, one cell per lane; the energy gathers take each pair from
+            // the side that owns it, so they stage only the home cell and its forward half
+            // shell
+            constexpr bool kOwnedOnly = MODE == M_U || MODE == M_DU;
             int nst = 0, ncnt = 0;
-            if (lane < 27) {
+            if (lane < 27 && (!kOwnedOnly || lane == 13 || ((fwd >> lane) & 1u))) {
                 int X0 = kx + lane % 3 - 1, Y0 = ky + (lane / 3) % 3 - 1, Z0 = kz + lane / 9 - 1;
                 X0 = X0 < 0 ? X0 + nx : (X0 >= nx ? X0 - nx : X0);
                 Y0 = Y0 < 0 ? Y0 + ny : (Y0 >= ny ? Y0 - ny : Y0);
@@ -254,6 +277,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                 hb.type = 0;
                 if (act) hb = sb[s0 + h0 + hi];
                 const int self = selfOff + h0 + hi;
+                const int hslot = s0 + h0 + hi;
                 const uint32_t me = act ? sidx[s0 + h0 + hi] : 0u;
                 const float hfx = (float)(hb.x - ox0), hfy = (float)(hb.y - oy0), hfz = (float)(hb.z - oz0);
                 float fx = 0.f, fy = 0.f, fz = 0.f;
@@ -309,7 +333,9 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                             cd.z = qb[it].z;
                         }
                         cd.t = qb[it].type;
-                        cd.c = kF ? 0 : (c | ((int)src[it] << 5));
+                        // force: the candidate's sorted slot (the self test reads it with the
+                        // type, in the same 16-byte load); gathers: stencil cell | slot << 5
+                        cd.c = kF ? src[it] : (c | ((int)src[it] << 5));
                         Cw[lane + it * kWave] = cd;
                     }
                     // per half of the home cell, the staged candidates it can reach
@@ -330,24 +356,29 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         // one phase over the home's half list: FP64 test, force law under
                         // the branch; two candidates per trip so both LDS reads are in
                         // flight before either is used
-                        const float *Cf = sCf + 6 * hb.type * nT;
+                        const float4 *C8 = sC8 + 2 * hb.type * nT;
                         const int half = hfx >= 0.f ? 1 : 0;
                         const uint8_t *lst = sHalf[kF1 ? wave : 0][half];
                         const int nl = half ? nl1 : nl0;
+                        // the next trip's list entries are read one trip ahead (clamped
+                        // into the list; entries past nl are never used)
                         int j = sub;
+                        int i0 = lst[min(j, kMaxC - 1)], i1 = lst[min(j + tpp, kMaxC - 1)];
                         for (; j + tpp < nl; j += 2 * tpp) {
-                            const int i0 = lst[j], i1 = lst[j + tpp];
-                            const Cand q0 = Cw[i0];
-                            const Cand q1 = Cw[i1];
-                            force_term(hb, q0, cb + i0 != self, cut2, Cf, fx, fy, fz);
-                            force_term(hb, q1, cb + i1 != self, cut2, Cf, fx, fy, fz);
+                            const int n0 = lst[min(j + 2 * tpp, kMaxC - 1)], n1 = lst[min(j + 3 * tpp, kMaxC - 1)];
+                            const Cand q0 = load_cand(Cw + i0);
+                            const Cand q1 = load_cand(Cw + i1);
+                            force_term(hb, q0, q0.c != hslot, cut2, C8, fx, fy, fz);
+                            force_term(hb, q1, q1.c != hslot, cut2, C8, fx, fy, fz);
+                            i0 = n0;
+                            i1 = n1;
                         }
                         if (j < nl) {
-                            const int i0 = lst[j];
-                            force_term(hb, Cw[i0], cb + i0 != self, cut2, Cf, fx, fy, fz);
+                            const Cand q0 = load_cand(Cw + i0);
+                            force_term(hb, q0, q0.c != hslot, cut2, C8, fx, fy, fz);
                         }
                     } else if (k2P) {
-                        const float *Cf = sCf + 6 * hb.type * nT;
+                        const float4 *C8 = sC8 + 2 * hb.type * nT;
                         uint8_t *lst = &sList[k2P ? wave : 0][0][lane];
                         for (int jb = cb + sub; __ballot(act && jb < ce); jb += tpp * kList) {
                             // phase 1: FP32 screen, hits kept in the lane's list
@@ -369,7 +400,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                             for (int o = 1; o < kWave; o <<= 1) most = max(most, __shfl_xor(most, o));
                             // phase 2: exact FP64 test and the force law on the hits
                             for (int kk = 0; kk < most; kk++)
-                                if (kk < nhit) force_term(hb, Cw[lst[kWave * kk]], true, cut2, Cf, fx, fy, fz);
+                                if (kk < nhit) force_term(hb, load_cand(Cw + lst[kWave * kk]), true, cut2, C8, fx, fy, fz);
                         }
                     } else if (!kF && act) {
                         for (int j = cb + sub; j < ce; j += tpp) {

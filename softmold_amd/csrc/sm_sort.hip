@@ -29,6 +29,7 @@ constexpr int kRounds = 4;
 constexpr int kTile = kThreads * kRounds;
 constexpr uint32_t kFlagAgg = 1u << 30, kFlagIncl = 2u << 30, kCountMask = (1u << 30) - 1;
 constexpr int kSpinLimit = 1 << 20;
+constexpr int kLook = 8;
 
 __global__ __launch_bounds__(kThreads) void k_onesweep(int n, const uint32_t *__restrict__ kin,
                                                        const uint32_t *__restrict__ vin,
@@ -71,26 +72,38 @@ __global__ __launch_bounds__(kThreads) void k_onesweep(int n, const uint32_t *__
         for (int w = 0; w < kWavesPerBlock; w++) mine += wc[r][w][t];
     uint32_t *st = status + (size_t)tile * 256 + t;
     __hip_atomic_store(st, (tile == 0 ? kFlagIncl : kFlagAgg) | mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // Look-back over a window of kLook predecessor tiles at a time: the window's loads are
+    // in flight together, so a walk over many aggregate-only tiles costs one memory round
+    // trip per kLook tiles.  Entries below tile 0 read as an inclusive zero.
     uint32_t prefix = 0;
+    int spins = 0;
     for (int pt = tile - 1; pt >= 0;) {
-        const uint32_t v = __hip_atomic_load(status + (size_t)pt * 256 + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t f = v & ~kCountMask;
-        if (f == 0u) {
-            int spins = 0;
-            uint32_t w = 0;
-            do {
-                __builtin_amdgcn_s_sleep(1);
-                w = __hip_atomic_load(status + (size_t)pt * 256 + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            } while ((w & ~kCountMask) == 0u && ++spins < kSpinLimit);
-            if ((w & ~kCountMask) == 0u) {
+        uint32_t v[kLook];
+#pragma unroll
+        for (int w = 0; w < kLook; w++)
+            v[w] = pt - w >= 0 ? __hip_atomic_load(status + (size_t)(pt - w) * 256 + t, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT)
+                               : kFlagIncl;
+        int used = 0;
+        bool done = false;
+#pragma unroll
+        for (int w = 0; w < kLook; w++) {
+            if (done || used < w) continue;    // stopped earlier in the window
+            const uint32_t f = v[w] & ~kCountMask;
+            if (f == 0u) continue;             // not published yet: resume here
+            prefix += v[w] & kCountMask;
+            used = w + 1;
+            done = f == kFlagIncl;
+        }
+        if (done) break;
+        pt -= used;
+        if (used < kLook) {                    // waiting on tile pt
+            if (++spins >= kSpinLimit) {
                 atomicOr(errflag, FLAG_SORT_TIMEOUT);
                 break;
             }
-            continue;
+            __builtin_amdgcn_s_sleep(1);
         }
-        prefix += v & kCountMask;
-        if (f == kFlagIncl) break;
-        pt--;
     }
     if (tile > 0)
         __hip_atomic_store(st, kFlagIncl | (prefix + mine), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -98,6 +98,12 @@ def test_cli_job_matches_library(tmp_path):
     assert pot[0, 1] == pytest.approx(s0.pair_potential() + s0.chain_potential(), rel=2e-6)
     np.testing.assert_allclose(size[:, 1:], np.tile(d["box"], (5, 1)), rtol=1e-6)
     np.testing.assert_allclose(temp[:, 1], d["temperature"])
+    # without BOND / BEND records MD writes lBond = 0/0 and zero bend sums
+    # (dataExtraction.h:1009-1027), in the text x86-64 prints for that NaN
+    lb = (tmp_path / "lBond_job.dat").read_text().splitlines()
+    be = (tmp_path / "bend_job.dat").read_text().splitlines()
+    assert [ln.split("\t") for ln in lb] == [[f"{t:g}", "-nan"] for t in (0, 0.2, 0.4, 0.6, 0.8)]
+    assert [ln.split("\t")[1:] for ln in be] == [["0", "0", "0"]] * 5
 
     frames = _frames(tmp_path / "frames_job.xyz")
     assert len(frames) == 3    # step 0, 20, 40

@@ -1,0 +1,212 @@
+"""Generic BOND and BEND records (MDcuda.cu:162-184; system.h:1881-2035, 2718-2830,
+3399-3530) on the oracle and on the device.
+
+A CHAIN of length L is, term for term, L-1 bonds (k, k+1) and L-2 bends (k, k+1, k+2) with
+the chain's constants (system.h:1782-1866).  Rewriting a system's CHAIN blocks as BOND and
+BEND records therefore leaves every force and energy term unchanged; only the order in
+which a bead's terms are summed differs.  The CHAIN path is pinned against the reference
+elsewhere, so this equivalence pins the generic path; finite differences and Newton's
+third law pin it on records that no CHAIN can express (bonds across lipids).
+"""
+import numpy as np
+import pytest
+
+import softmold_amd as sm
+from conftest import gpu_available
+import oracle as O
+
+
+def _records(d):
+    """The CHAIN blocks of system dict d as BOND and BEND records."""
+    bonds, bc, bends, ec = [], [], [], []
+    for (start, nch, length), c in zip(d["chain"], d["chain_const"]):
+        for k in range(start, start + nch * length, length):
+            for j in range(length - 1):
+                bonds.append((k + j, k + j + 1))
+                bc.append((c[0], c[1]))
+            for j in range(length - 2):
+                bends.append((k + j, k + j + 1, k + j + 2))
+                ec.append((c[2], c[3]))
+    e = dict(d)
+    e["chain"] = np.zeros((0, 3), np.int32)
+    e["chain_const"] = np.zeros((0, 4))
+    e["bond"], e["bond_const"] = np.array(bonds, np.int32), np.array(bc)
+    e["bend"], e["bend_const"] = np.array(bends, np.int32), np.array(ec)
+    return e
+
+
+def _crosslinked(d, n_links=60, seed=3):
+    """d plus BOND records between the heads of random lipid pairs (l0 1.5, k 20) and BEND
+    records over random head triples: terms no CHAIN record can express."""
+    rng = np.random.default_rng(seed)
+    heads = np.flatnonzero(d["type"] == 2)
+    e = dict(d)
+    b = rng.choice(heads, size=(n_links, 2), replace=True)
+    b = b[b[:, 0] != b[:, 1]]
+    t = rng.choice(heads, size=(n_links // 2, 3), replace=True)
+    t = t[(t[:, 0] != t[:, 1]) & (t[:, 1] != t[:, 2]) & (t[:, 0] != t[:, 2])]
+    e["bond"], e["bond_const"] = b.astype(np.int32), np.tile([1.5, 20.0], (len(b), 1))
+    e["bend"], e["bend_const"] = t.astype(np.int32), np.tile([0.5, 10.0], (len(t), 1))
+    return e
+
+
+def _oracle(d, pos=None):
+    return O.System(d["pos"] if pos is None else pos, d["type"], d["vel"], d["box"], d["fconst"], d["uconst"],
+                    d["n_types"], d["chain"], d["chain_const"], cutoff=d["cutoff"], dt=d["dt"],
+                    gamma=d["gamma"], temp=d["temperature"], delta_lxy=d["delta_lxy"], tension=d["tension"],
+                    seed=d["seed"], bonds=d.get("bond"), bond_const=d.get("bond_const"), bends=d.get("bend"),
+                    bend_const=d.get("bend_const"))
+
+
+def _deformed(d, seed=7, amp=0.15):
+    rng = np.random.default_rng(seed)
+    return np.mod(d["pos"] + rng.uniform(-amp, amp, d["pos"].shape), d["box"])
+
+
+# ---- oracle (CPU) --------------------------------------------------------------------
+
+def test_oracle_records_equal_chains():
+    d = sm.Mpd.liposome(5, 300, 3.45).system()
+    pos = _deformed(d)
+    a, b = _oracle(d, pos), _oracle(_records(d), pos)
+    fa, fb = a.chain_force(), b.chain_force()
+    assert np.abs(fa - fb).max() <= 1e-12 * np.abs(fa).max()
+    assert b.chain_potential() == pytest.approx(a.chain_potential(), rel=1e-13)
+    s = [1.003, 0.998, 1.0 / (1.003 * 0.998)]
+    assert b.chain_dpotential(s) == pytest.approx(a.chain_dpotential(s), rel=1e-10)
+
+
+def test_oracle_records_obey_newton_and_gradient():
+    d = _crosslinked(sm.Mpd.liposome(5, 300, 3.45).system())
+    d["chain"] = np.zeros((0, 3), np.int32)       # the generic records alone
+    d["chain_const"] = np.zeros((0, 4))
+    pos = _deformed(d)
+    o = _oracle(d, pos)
+    f = o.chain_force()
+    assert np.abs(f.sum(0)).max() <= 1e-10 * np.abs(f).max()
+    # -dU/dx by central differences on a few particles of the records
+    h = 1e-6
+    for i in (int(d["bond"][0, 0]), int(d["bend"][0, 1]), int(d["bend"][1, 2])):
+        for k in range(3):
+            p1, p2 = pos.copy(), pos.copy()
+            p1[i, k] += h
+            p2[i, k] -= h
+            du = (_oracle(d, p1).chain_potential() - _oracle(d, p2).chain_potential()) / (2 * h)
+            assert -du == pytest.approx(f[i, k], rel=1e-5, abs=1e-6)
+
+
+def test_oracle_bond_closed_form():
+    """One bond stretched to r: U = k (r - l0)^2 / 2, |F| = k (r - l0) along the bond."""
+    d = sm.Mpd.liposome(5, 300, 3.45).system()
+    d = dict(d)
+    d["chain"] = np.zeros((0, 3), np.int32)
+    d["chain_const"] = np.zeros((0, 4))
+    pos = d["pos"].copy()
+    pos[1] = pos[0] + np.array([0.9, 0.0, 0.0])
+    d["bond"], d["bond_const"] = np.array([[0, 1]], np.int32), np.array([[0.7, 100.0]])
+    o = _oracle(d, pos)
+    assert o.chain_potential() == pytest.approx(0.5 * 100.0 * 0.2 ** 2, rel=1e-12)
+    f = o.chain_force()
+    # the stretched bond pulls particle 0 towards particle 1 (+x) and 1 towards 0
+    assert f[0] == pytest.approx([100.0 * 0.2, 0.0, 0.0], rel=1e-12, abs=1e-12)
+    assert f[1] == pytest.approx([-100.0 * 0.2, 0.0, 0.0], rel=1e-12, abs=1e-12)
+
+
+def test_invalid_records_are_refused():
+    """Checked on the host before any device call."""
+    d = sm.Mpd.liposome(5, 100, 3.45).system()
+    e = dict(d, bond=np.array([[0, 0]], np.int32), bond_const=np.array([[1.0, 1.0]]))
+    with pytest.raises(sm.SmError, match="repeats"):
+        sm.Solver.from_system(e)
+    e = dict(d, bond=np.array([[0, len(d["type"])]], np.int32), bond_const=np.array([[1.0, 1.0]]))
+    with pytest.raises(sm.SmError, match="out of range"):
+        sm.Solver.from_system(e)
+
+
+# ---- device ----------------------------------------------------------------------------
+
+
+@pytest.mark.gpu
+def test_device_records_match_oracle_and_chains():
+    if not gpu_available():
+        pytest.skip("no GPU")
+    base = sm.Mpd.liposome(9, 1500, 3.45).system()
+    pos = _deformed(base)
+    for name, d in (("records", _records(base)), ("crosslinked", _crosslinked(base))):
+        print(f"case: {name}", flush=True)
+        s = sm.Solver.from_system(d)
+        s.upload(pos=pos)
+        _, fc = s.compute_forces()
+        o = _oracle(d, pos)
+        want = o.chain_force()
+        err = np.linalg.norm(fc - want, axis=1)
+        scale = np.maximum(np.linalg.norm(want, axis=1), 0.05 * np.sqrt((want ** 2).sum(1).mean()))
+        assert (err / scale).max() <= 1e-5, name
+        U = s.measure()
+        assert U[1] == pytest.approx(o.chain_potential(), rel=1e-10, abs=1e-9), name
+        sc = np.array([1.002, 0.999, 1.0 / (1.002 * 0.999)])
+        dup, duc = s.potential_delta(sc)
+        assert duc == pytest.approx(o.chain_dpotential(sc), rel=1e-9, abs=1e-9), name
+        s.close()
+
+
+@pytest.mark.gpu
+def test_device_records_trajectory_equals_chain_trajectory_closely():
+    """Steps with CHAIN blocks and with the same terms as records: the FP64 bonded forces
+    differ only in summation order, so the FP32 accelerations agree to rounding and the
+    trajectories stay together over a short run."""
+    if not gpu_available():
+        pytest.skip("no GPU")
+    d = sm.Mpd.liposome(9, 1500, 3.45).system()
+    a, b = sm.Solver.from_system(d), sm.Solver.from_system(_records(d))
+    for s in (a, b):
+        s.init_forces()
+        s.run(1, 20)
+    pa, pb = a.download()[0], b.download()[0]
+    dx = np.abs(pa - pb)
+    dx = np.minimum(dx, d["box"] - dx)
+    assert dx.max() < 1e-3
+    ua, ub = a.measure(), b.measure()
+    assert ub == pytest.approx(ua, rel=1e-4)
+
+
+@pytest.mark.gpu
+def test_batch_with_records_equals_solo():
+    if not gpu_available():
+        pytest.skip("no GPU")
+    x = _crosslinked(sm.Mpd.bilayer(11, 400, 3.45).system())
+    y = sm.Mpd.bilayer(12, 300, 3.45).system()
+    x.update(delta_lxy=0.05, tension=0.5)
+    y.update(delta_lxy=0.05)
+    bt = sm.Solver.batch([x, y])
+    bt.init_forces()
+    bt.run(1, 17)
+    bp = bt.download()[0]
+    bm = bt.measure()
+    for r, d in enumerate((x, y)):
+        s = sm.Solver.from_system(d)
+        s.init_forces()
+        s.run(1, 17)
+        lo, hi = bt.first[r], bt.first[r] + bt.count[r]
+        assert np.array_equal(s.download()[0], bp[lo:hi]), r
+        assert np.array_equal(s.measure(), bm[r]), r
+        s.close()
+
+
+@pytest.mark.gpu
+def test_device_bond_stats_match_oracle():
+    if not gpu_available():
+        pytest.skip("no GPU")
+    base = sm.Mpd.liposome(9, 1500, 3.45).system()
+    chain_only = sm.Solver.from_system(base)
+    st = chain_only.bond_stats()
+    assert np.isnan(st[0]) and list(st[1:]) == [0.0, 0.0, 0.0]     # no records: 0/0 and sums of nothing
+    chain_only.close()
+    d = _crosslinked(base)
+    s = sm.Solver.from_system(d)
+    s.init_forces()
+    s.run(1, 30)
+    pos = s.download()[0]
+    want = O.bond_stats(pos, d["box"], d["bond"], d["bend"])
+    assert s.bond_stats() == pytest.approx(want, rel=1e-12)
+    s.close()

@@ -88,11 +88,50 @@ def test_unsupported_molecules_are_refused(tmp_path):
     m = sm.Mpd.liposome(1, 50, 3.45)
     m.write(str(tmp_path / "g"))
     text = (tmp_path / "g.mpd").read_text()
-    # turn the CHAIN record into a BOND record over the first two beads
+    # turn the CHAIN record into a BOND record over the first two beads (supported)
     text = re.sub(r"\nmolecule \n8\t 1\n 0.7  100  1  100 \n 0  50  3 \n",
                   "\nmolecule \n6\t 1\n 0.7  100 \n 0  1 \n", text)
     (tmp_path / "b.mpd").write_text(text)
     b = sm.Mpd.read(str(tmp_path / "b"))
+    d = b.system()
+    assert d["bond"].tolist() == [[0, 1]] and d["bond_const"].tolist() == [[0.7, 100.0]]
+    assert len(d["chain"]) == 0 and len(d["bend"]) == 0
+    # a SOLID record (molecules.h:20) has no implementation here
+    text = re.sub(r"\nmolecule \n6\t 1\n 0.7  100 \n 0  1 \n", "\nmolecule \n10\t 1\n 1 \n 0 \n", text)
+    (tmp_path / "s.mpd").write_text(text)
+    b = sm.Mpd.read(str(tmp_path / "s"))
     with pytest.raises(sm.SmError) as e:
         b.system()
     assert e.value.code == sm.SM_ERR_UNSUPPORTED
+
+
+def _gamma_type_mpd(tmp_path, name, gammas):
+    """A generated vesicle whose `gamma` line is replaced by a gammaType line."""
+    m = sm.Mpd.liposome(4, 60, 3.45)
+    m.write(str(tmp_path / "src"))
+    m.close()
+    text = (tmp_path / "src.mpd").read_text()
+    line = "gammaType " + " ".join(f"{g:g}" for g in gammas) + " \n" if gammas else ""
+    text = text.replace("gamma 1 \n", "").replace("nTypes 6 \n", "nTypes 6 \n" + line)
+    (tmp_path / f"{name}.mpd").write_text(text)
+
+
+def test_gamma_type_thermostat_view(tmp_path):
+    """Without a scalar gamma, MD.cpp runs the gammaType thermostat (MD.cpp:135-138), whose
+    compute() applies gammaType[0] to every particle (langevin.h:254) with sigma taken at the
+    first temperature (langevin.h:235-241)."""
+    _gamma_type_mpd(tmp_path, "t", [0.5, 2.0, 3.0, 4.0, 5.0, 6.0])
+    d = sm.Mpd.read(str(tmp_path / "t")).system()
+    assert d["gamma"] == 0.5 and d["sigma_temperature"] == 3.0
+    d = sm.Mpd.liposome(4, 60, 3.45).system()
+    assert d["gamma"] == 1.0 and d["sigma_temperature"] == 0.0
+
+
+def test_no_gamma_stops_like_md(tmp_path):
+    _gamma_type_mpd(tmp_path, "n", [])
+    with pytest.raises(sm.SmError, match="No gamma available"):
+        sm.Mpd.read(str(tmp_path / "n")).system()
+    cli = os.path.join(os.path.dirname(sm.__file__), "md_hip")
+    import subprocess
+    r = subprocess.run([cli, "n"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and r.stdout == "Error(main): No gamma available!\n"
