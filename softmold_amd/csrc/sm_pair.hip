@@ -182,8 +182,9 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     constexpr bool kF = MODE == M_FORCE || MODE == M_FORCE1;   // force (two-phase / one-phase)
     constexpr bool k2P = MODE == M_FORCE;
     constexpr bool kF1 = MODE == M_FORCE1;
-    constexpr int kMaxC = k2P ? 128 : 256;   // staged candidates per pass
+    constexpr int kMaxC = k2P ? 128 : 192;   // staged candidates per pass (192 measured faster than 256)
     constexpr int kStageIters = kMaxC / kWave;
+    static_assert(kMaxC % kWave == 0, "staging rounds cover whole waves");
     __shared__ float4 sC8[kF ? 2 * kMaxTypes * kMaxTypes : 1];   // rows padded to 8 floats
     __shared__ double sCd[kF ? 1 : 6 * kMaxTypes * kMaxTypes];
     __shared__ Cand sCand[kWaves][kMaxC];
@@ -261,9 +262,6 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
             const int excl = incl - ncnt;
             const int total = __builtin_amdgcn_readlane(incl, 26);
             const int selfOff = __builtin_amdgcn_readlane(excl, 13);
-            int off[26];
-#pragma unroll
-            for (int c = 0; c < 26; c++) off[c] = __builtin_amdgcn_readlane(incl, c);
             const int nh = e0 - s0;
 
             for (int h0 = 0; h0 < nh; h0 += kWave) {
@@ -285,14 +283,24 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                 int cnt = 0;
                 for (int cb = 0; cb < total; cb += kMaxC) {
                     const int ce = min(total, cb + kMaxC);
+                    // staging rounds this pass needs (wave-uniform): the unrolled rounds
+                    // past it are skipped, so a short last pass costs what it holds
+                    const int iters = (ce - cb + kWave - 1) / kWave;
                     // stage [cb, ce): addresses first, then all loads, then the LDS stores
                     int src[kStageIters], cel[kStageIters];
 #pragma unroll
                     for (int it = 0; it < kStageIters; it++) {
+                        src[it] = -1;
+                        cel[it] = 0;
+                        if (it >= iters) continue;
                         const int t = cb + lane + it * kWave;
+                        // the stencil cell holding slot t: the number of cells whose
+                        // inclusive end is <= t, by binary search over lanes 0..31 of the
+                        // scan (lanes 27..31 hold `total`, above every t)
                         int c = 0;
 #pragma unroll
-                        for (int q = 0; q < 26; q++) c += (t >= off[q]) ? 1 : 0;
+                        for (int step = 16; step > 0; step >>= 1)
+                            c += (__shfl(incl, c + step - 1) <= t) ? step : 0;
                         const int stc = __shfl(nst, c), exc = __shfl(excl, c);
                         cel[it] = c;
                         src[it] = (t < ce) ? stc + (t - exc) : -1;
@@ -300,12 +308,12 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                     Bead qb[kStageIters];
 #pragma unroll
                     for (int it = 0; it < kStageIters; it++)
-                        if (src[it] >= 0) qb[it] = sb[src[it]];
+                        if (it < iters && src[it] >= 0) qb[it] = sb[src[it]];
                     int reach[kStageIters];
 #pragma unroll
                     for (int it = 0; it < kStageIters; it++) {
                         reach[it] = 0;
-                        if (src[it] < 0) continue;
+                        if (it >= iters || src[it] < 0) continue;
                         const int c = cel[it];
                         Cand cd;
                         if (kF) {
@@ -343,6 +351,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                     if (MODE == M_FORCE1) {
 #pragma unroll
                         for (int it = 0; it < kStageIters; it++) {
+                            if (it >= iters) break;
                             const bool r0 = reach[it] & 1, r1 = reach[it] & 2;
                             const unsigned long long b0 = __ballot(r0), b1 = __ballot(r1);
                             if (r0) sHalf[kF1 ? wave : 0][0][nl0 + __popcll(b0 & lanes_lt)] = (uint8_t)(lane + it * kWave);
