@@ -86,20 +86,36 @@ __device__ inline double scaled_diff(double a, double b, double s)
     return a * s - b * s;
 }
 
-// Potential<double> (MD.h:895-930)
-__device__ inline double pair_u(double r2, const double *C)
+
+// sqrt of a positive, normal double: v_rsq_f64 and the two Goldschmidt corrections the
+// device library uses, without its range scaling and special-value fix-ups (r^2 here lies
+// in (0, 4 rc^2]); 0 maps to 0.
+__device__ inline double sqrt_pos(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double r = x * y, h = 0.5 * y;
+    const double e = fma(-h, r, 0.5);
+    r = fma(r, e, r);
+    h = fma(h, e, h);
+    double d = fma(-r, r, x);
+    r = fma(d, h, r);
+    d = fma(-r, r, x);
+    r = fma(d, h, r);
+    return x > 0.0 ? r : 0.0;
+}
+
+// Potential<double> (MD.h:895-930) with both branches evaluated and one selected, so a
+// wave does not run the two paths one after the other; each branch keeps the reference's
+// operation order.
+__device__ inline double pair_u_any(double r2, const double *C)
 {
 #pragma clang fp contract(off)
-    const double dr = sqrt(r2);
-    double pot;
-    if (dr <= C[0]) {
-        pot = C[0] - dr;
-        pot = C[1] * pot * pot + C[2];
-    } else {
-        pot = C[3] - dr;
-        pot = pot * pot * (C[4] - pot * C[5]);
-    }
-    return pot;
+    const double dr = sqrt_pos(r2);
+    const double a = C[0] - dr;
+    const double inner = C[1] * a * a + C[2];
+    const double b = C[3] - dr;
+    const double outer = b * b * (C[4] - b * C[5]);
+    return dr <= C[0] ? inner : outer;
 }
 
 // Forward half shell of the reference (cellOpt.h:715) over stencil index
@@ -182,6 +198,8 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     constexpr bool kF = MODE == M_FORCE || MODE == M_FORCE1;   // force (two-phase / one-phase)
     constexpr bool k2P = MODE == M_FORCE;
     constexpr bool kF1 = MODE == M_FORCE1;
+    constexpr bool kE = MODE == M_U || MODE == M_DU;   // FP64 energy gathers
+    constexpr bool kCull = kF1 || kE;                  // candidates culled per half of the home cell
     constexpr int kMaxC = k2P ? 128 : 192;   // staged candidates per pass (192 measured faster than 256)
     constexpr int kStageIters = kMaxC / kWave;
     static_assert(kMaxC % kWave == 0, "staging rounds cover whole waves");
@@ -190,7 +208,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     __shared__ Cand sCand[kWaves][kMaxC];
     __shared__ float4 sRel[k2P ? kWaves : 1][k2P ? kMaxC : 1];
     __shared__ uint8_t sList[k2P ? kWaves : 1][k2P ? kList : 1][kWave];   // per-lane hit lists, lane-minor
-    __shared__ uint8_t sHalf[kF1 ? kWaves : 1][2][kF1 ? kMaxC : 1];         // per half-cell candidate lists
+    __shared__ uint8_t sHalf[kCull ? kWaves : 1][2][kCull ? kMaxC : 1];     // per half-cell candidate lists
     __shared__ float redf[kF ? kWaves : 1][3][kWave];
     __shared__ double redd[kF ? 1 : kWaves][kWave];
     __shared__ int redi[kWaves][kWave];
@@ -231,6 +249,10 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
             const double csx = R.cs[0], csy = R.cs[1], csz = R.cs[2];
             const float hcx = (float)(0.5 * csx), hcy = (float)(0.5 * csy), hcz = (float)(0.5 * csz);
             const double ax = R.scale[0], ay = R.scale[1], az = R.scale[2];
+            // energy gathers: a candidate farther than (rc + margin) / min(1, scale) from the
+            // home half-cell is beyond rc both before and after the trial scaling
+            const double smin = fmin(1.0, fmin(ax, fmin(ay, az)));
+            const float reachE = MODE == M_DU ? (float)((double)reach2 / (smin * smin)) : reach2;
             const uint32_t kbase = (uint32_t)R.cellbase;
             const uint32_t k = kg - kbase;
             const int kx = (int)(k % (uint32_t)nx);
@@ -316,30 +338,28 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         if (it >= iters || src[it] < 0) continue;
                         const int c = cel[it];
                         Cand cd;
-                        if (kF) {
-                            const int X0 = kx + c % 3 - 1, Y0 = ky + (c / 3) % 3 - 1, Z0 = kz + c / 9 - 1;
-                            cd.x = qb[it].x + (X0 < 0 ? -Lx : (X0 >= nx ? Lx : 0.0));
-                            cd.y = qb[it].y + (Y0 < 0 ? -Ly : (Y0 >= ny ? Ly : 0.0));
-                            cd.z = qb[it].z + (Z0 < 0 ? -Lz : (Z0 >= nz ? Lz : 0.0));
-                            if (k2P)
-                                Rw[lane + it * kWave] = make_float4((float)(cd.x - ox0), (float)(cd.y - oy0),
-                                                                    (float)(cd.z - oz0), 0.f);
-                            if (MODE == M_FORCE1) {
-                                // which half of the home cell (x < centre, x >= centre) the
-                                // candidate can reach: box distance below rc (+ margin)
-                                const float rx = (float)(cd.x - ox0), ry = (float)(cd.y - oy0),
-                                            rz = (float)(cd.z - oz0);
-                                const float gy = fmaxf(fabsf(ry) - hcy, 0.f), gz = fmaxf(fabsf(rz) - hcz, 0.f);
-                                const float base = gy * gy + gz * gz;
-                                const float g0 = fmaxf(fmaxf(rx, -hcx - rx), 0.f);
-                                const float g1 = fmaxf(fmaxf(-rx, rx - hcx), 0.f);
-                                reach[it] = (base + g0 * g0 < reach2 ? 1 : 0) | (base + g1 * g1 < reach2 ? 2 : 0);
-                            }
-                        } else {
-                            cd.x = qb[it].x;
-                            cd.y = qb[it].y;
-                            cd.z = qb[it].z;
+                        const int X0 = kx + c % 3 - 1, Y0 = ky + (c / 3) % 3 - 1, Z0 = kz + c / 9 - 1;
+                        const double ix = qb[it].x + (X0 < 0 ? -Lx : (X0 >= nx ? Lx : 0.0));
+                        const double iy = qb[it].y + (Y0 < 0 ? -Ly : (Y0 >= ny ? Ly : 0.0));
+                        const double iz = qb[it].z + (Z0 < 0 ? -Lz : (Z0 >= nz ? Lz : 0.0));
+                        if (k2P)
+                            Rw[lane + it * kWave] = make_float4((float)(ix - ox0), (float)(iy - oy0), (float)(iz - oz0), 0.f);
+                        if (kCull) {
+                            // which half of the home cell (x < centre, x >= centre) the
+                            // candidate can reach: box distance below the reach radius
+                            const float rx = (float)(ix - ox0), ry = (float)(iy - oy0), rz = (float)(iz - oz0);
+                            const float gy = fmaxf(fabsf(ry) - hcy, 0.f), gz = fmaxf(fabsf(rz) - hcz, 0.f);
+                            const float base = gy * gy + gz * gz;
+                            const float g0 = fmaxf(fmaxf(rx, -hcx - rx), 0.f);
+                            const float g1 = fmaxf(fmaxf(-rx, rx - hcx), 0.f);
+                            const float r2max = kE ? reachE : reach2;
+                            reach[it] = (base + g0 * g0 < r2max ? 1 : 0) | (base + g1 * g1 < r2max ? 2 : 0);
                         }
+                        // forces stage imaged positions; gathers the stored ones (the image
+                        // shift is applied per pair, as the reference's minImg)
+                        cd.x = kF ? ix : qb[it].x;
+                        cd.y = kF ? iy : qb[it].y;
+                        cd.z = kF ? iz : qb[it].z;
                         cd.t = qb[it].type;
                         // force: the candidate's sorted slot (the self test reads it with the
                         // type, in the same 16-byte load); gathers: stencil cell | slot << 5
@@ -348,14 +368,14 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                     }
                     // per half of the home cell, the staged candidates it can reach
                     int nl0 = 0, nl1 = 0;
-                    if (MODE == M_FORCE1) {
+                    if (kCull) {
 #pragma unroll
                         for (int it = 0; it < kStageIters; it++) {
                             if (it >= iters) break;
                             const bool r0 = reach[it] & 1, r1 = reach[it] & 2;
                             const unsigned long long b0 = __ballot(r0), b1 = __ballot(r1);
-                            if (r0) sHalf[kF1 ? wave : 0][0][nl0 + __popcll(b0 & lanes_lt)] = (uint8_t)(lane + it * kWave);
-                            if (r1) sHalf[kF1 ? wave : 0][1][nl1 + __popcll(b1 & lanes_lt)] = (uint8_t)(lane + it * kWave);
+                            if (r0) sHalf[kCull ? wave : 0][0][nl0 + __popcll(b0 & lanes_lt)] = (uint8_t)(lane + it * kWave);
+                            if (r1) sHalf[kCull ? wave : 0][1][nl1 + __popcll(b1 & lanes_lt)] = (uint8_t)(lane + it * kWave);
                             nl0 += __popcll(b0);
                             nl1 += __popcll(b1);
                         }
@@ -367,7 +387,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         // flight before either is used
                         const float4 *C8 = sC8 + 2 * hb.type * nT;
                         const int half = hfx >= 0.f ? 1 : 0;
-                        const uint8_t *lst = sHalf[kF1 ? wave : 0][half];
+                        const uint8_t *lst = sHalf[kCull ? wave : 0][half];
                         const int nl = half ? nl1 : nl0;
                         // the next trip's list entries are read one trip ahead (clamped
                         // into the list; entries past nl are never used)
@@ -411,7 +431,38 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                             for (int kk = 0; kk < most; kk++)
                                 if (kk < nhit) force_term(hb, load_cand(Cw + lst[kWave * kk]), true, cut2, C8, fx, fy, fz);
                         }
+                    } else if (kE && act) {
+                        // each pair once, from the side that owns it: only the home cell and
+                        // its forward half shell are staged, and within the home cell the
+                        // lower slot owns the pair
+                        const int half = hfx >= 0.f ? 1 : 0;
+                        const uint8_t *lst = sHalf[kCull ? wave : 0][half];
+                        const int nl = half ? nl1 : nl0;
+                        const double *Ch = sCd + 6 * hb.type * nT;
+                        for (int jj = sub; jj < nl; jj += tpp) {
+                            const int i = lst[jj];
+                            const int j = cb + i;
+                            const Cand q = Cw[i];
+                            const int c = q.c & 31;
+                            if (j == self || (c == 13 && j < self)) continue;
+                            const int X0 = kx + c % 3 - 1, Y0 = ky + (c / 3) % 3 - 1, Z0 = kz + c / 9 - 1;
+                            const double px = q.x + (X0 < 0 ? -Lx : (X0 >= nx ? Lx : 0.0));
+                            const double py = q.y + (Y0 < 0 ? -Ly : (Y0 >= ny ? Ly : 0.0));
+                            const double pz = q.z + (Z0 < 0 ? -Lz : (Z0 >= nz ? Lz : 0.0));
+                            const double *C = Ch + 6 * q.t;
+                            const double r2 = sq3(hb.x - px, hb.y - py, hb.z - pz);
+                            if (MODE == M_U) {
+                                accu += r2 < cut2 ? pair_u_any(r2, C) : 0.0;
+                            } else {
+                                const double r2n = sq3(scaled_diff(hb.x, px, ax), scaled_diff(hb.y, py, ay),
+                                                       scaled_diff(hb.z, pz, az));
+                                const double oldP = r2 < cut2 ? pair_u_any(r2, C) : 0.0;
+                                const double newP = r2n < cut2 ? pair_u_any(r2n, C) : 0.0;
+                                accu += (oldP - newP);
+                            }
+                        }
                     } else if (!kF && act) {
+                        // neighbour counts and pair lists see both sides of every pair
                         for (int j = cb + sub; j < ce; j += tpp) {
                             if (j == self) continue;
                             const Cand q = Cw[j - cb];
@@ -421,9 +472,6 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                             const double sy = Y0 < 0 ? -Ly : (Y0 >= ny ? Ly : 0.0);
                             const double sz = Z0 < 0 ? -Lz : (Z0 >= nz ? Lz : 0.0);
                             const bool own = (c == 13) || ((fwd >> c) & 1u);
-                            // energies: each pair once, from the side that owns it (within
-                            // one cell, from the lower slot); counts and lists see both sides
-                            if ((MODE == M_U || MODE == M_DU) && (!own || (c == 13 && j < self))) continue;
                             double ox, oy, oz, px, py, pz;
                             if (own) {
                                 ox = hb.x; oy = hb.y; oz = hb.z;
@@ -433,18 +481,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                                 px = hb.x - sx; py = hb.y - sy; pz = hb.z - sz;
                             }
                             const double r2 = sq3(ox - px, oy - py, oz - pz);
-                            if (MODE == M_U || MODE == M_DU) {
-                                const double *C = sCd + 6 * ((own ? hb.type : q.t) * nT + (own ? q.t : hb.type));
-                                if (MODE == M_U) {
-                                    if (r2 < cut2) accu += pair_u(r2, C);
-                                } else {
-                                    const double oldP = (r2 < cut2) ? pair_u(r2, C) : 0.0;
-                                    const double r2n = sq3(scaled_diff(ox, px, ax), scaled_diff(oy, py, ay),
-                                                           scaled_diff(oz, pz, az));
-                                    const double newP = (r2n < cut2) ? pair_u(r2n, C) : 0.0;
-                                    accu += (oldP - newP);
-                                }
-                            } else if (r2 < cut2) {
+                            if (r2 < cut2) {
                                 cnt++;
                                 if (MODE == M_LIST) {
                                     const uint32_t other = sidx[q.c >> 5];
@@ -513,9 +550,11 @@ void launch_cells(hipStream_t st, const Params &P, int chunk, const Bead *sbeads
     // With at least 3 cells per axis, a cell is shorter than 4/3 of the cutoff.
     const double csmax = P.cutoff * 4.0 / 3.0;
     const double slack = 64.0 * P.cutoff * 1.5 * csmax * (1.0 / 8388608.0) + 1e-6 * cut2;
-    // one-phase force: the half-cell culling radius, rc plus a margin far above the FP32
-    // rounding of cell-relative coordinates (|r| < 3 cells) and the cell-assignment rounding
-    const float cut2pre = MODE == M_FORCE1 ? (float)((P.cutoff + 1e-4) * (P.cutoff + 1e-4)) : (float)(cut2 + slack);
+    // culled modes (one-phase force, energy gathers): the half-cell culling radius, rc plus a
+    // margin far above the FP32 rounding of cell-relative coordinates (|r| < 3 cells) and the
+    // cell-assignment rounding; the dU gather widens it by its trial scale in the kernel
+    const bool culled = MODE == M_FORCE1 || MODE == M_U || MODE == M_DU;
+    const float cut2pre = culled ? (float)((P.cutoff + 1e-4) * (P.cutoff + 1e-4)) : (float)(cut2 + slack);
     hipLaunchKernelGGL(k_cells<MODE>, dim3(nb), dim3(kWaves * kWave), 0, st, P.n, P.nT, chunk, nb, cut2, cut2pre,
                        sbeads, skey, sidx, cstart, cend, reps, fconst, uconst, out);
 }
