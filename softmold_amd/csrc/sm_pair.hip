@@ -10,9 +10,9 @@
 //   1. lanes 0..26 look up the 27 stencil cells in the dense cell table (validated
 //      against the sorted keys, so the table is never cleared);
 //   2. a wave scan of the 27 counts lays the candidates out contiguously and every lane
-//      stages its share into LDS (the stencil cell of an item is found by comparing
-//      against the 26 uniform offsets held in SGPRs), so all global loads of the stencil
-//      are in flight together;
+//      stages its share into LDS (the stencil cell of an item by a binary search over the
+//      scan's lanes), so all global loads of a pass are in flight together; a pass holds
+//      kMaxC candidates and runs only the 64-slot rounds it fills;
 //   3. tpp = 64/H lanes serve each of the H home particles, each lane walking every
 //      tpp-th candidate; partial results meet in LDS and are combined in lane order.
 // Force, one phase (M_FORCE1, the default): candidates are staged as FP64 imaged
@@ -27,7 +27,9 @@
 // widened by more than the FP32 error, keeping the hits in a per-lane LDS list, then runs
 // the exact FP64 test and the force law on the hits only.  Gathers: FP64 throughout, each pair's
 // difference taken in the reference's direction -- the particle of the cell that owns
-// the pair in the 13-cell half shell (cellOpt.h:715) minus the imaged other.
+// the pair in the 13-cell half shell (cellOpt.h:715) minus the imaged other.  The energy
+// gathers (U, dU) stage only the home cell and its forward half shell and use the same
+// half-cell culling, with the reach radius widened by the trial scale for dU.
 #include "sm_internal.hpp"
 
 namespace smdev {
@@ -208,7 +210,10 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     __shared__ Cand sCand[kWaves][kMaxC];
     __shared__ float4 sRel[k2P ? kWaves : 1][k2P ? kMaxC : 1];
     __shared__ uint8_t sList[k2P ? kWaves : 1][k2P ? kList : 1][kWave];   // per-lane hit lists, lane-minor
-    __shared__ uint8_t sHalf[kCull ? kWaves : 1][2][kCull ? kMaxC : 1];     // per half-cell candidate lists
+    // per half-cell candidate lists, padded so the force loop's reads one trip ahead (at most
+    // 3 x 64 entries past the end) stay inside the row
+    constexpr int kHalfRow = kMaxC + 3 * kWave;
+    __shared__ uint8_t sHalf[kCull ? kWaves : 1][2][kCull ? kHalfRow : 1];
     __shared__ float redf[kF ? kWaves : 1][3][kWave];
     __shared__ double redd[kF ? 1 : kWaves][kWave];
     __shared__ int redi[kWaves][kWave];
@@ -389,12 +394,14 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         const int half = hfx >= 0.f ? 1 : 0;
                         const uint8_t *lst = sHalf[kCull ? wave : 0][half];
                         const int nl = half ? nl1 : nl0;
-                        // the next trip's list entries are read one trip ahead (clamped
-                        // into the list; entries past nl are never used)
+                        // the next trip's list entries are read one trip ahead (entries past
+                        // nl fall in the row's padding and are never used)
                         int j = sub;
-                        int i0 = lst[min(j, kMaxC - 1)], i1 = lst[min(j + tpp, kMaxC - 1)];
+                        const uint8_t *lp = lst + sub;
+                        int i0 = lp[0], i1 = lp[tpp];
                         for (; j + tpp < nl; j += 2 * tpp) {
-                            const int n0 = lst[min(j + 2 * tpp, kMaxC - 1)], n1 = lst[min(j + 3 * tpp, kMaxC - 1)];
+                            lp += 2 * tpp;
+                            const int n0 = lp[0], n1 = lp[tpp];
                             const Cand q0 = load_cand(Cw + i0);
                             const Cand q1 = load_cand(Cw + i1);
                             force_term(hb, q0, q0.c != hslot, cut2, C8, fx, fy, fz);

@@ -154,7 +154,7 @@ def test_autograd_function():
     J = ConstantsObjective.apply(theta, s, PAIR, 16, 4)
     (2.0 * J).backward()
     Jr, gr = objective(s, PAIR, 16, 4)
-    assert float(J) == Jr
+    assert float(J.detach()) == Jr
     np.testing.assert_array_equal(theta.grad.numpy(), 2.0 * gr)
 
 
