@@ -58,6 +58,13 @@ typedef struct sm_system {
     int32_t n_bends;
     const int32_t *bend;        /* 3 per bend */
     const double *bend_const;   /* 2 per bend: cos-theta term, k_bend (bendF, MD.h:683-723) */
+    /* BALL records (doBallForce, system.h:1937-1971): a half-harmonic repulsion between the
+     * ball's centre particle and each member, active only inside r0 (harmonicHalfF,
+     * MD.h:1164-1187); U adds harmonicHalfP (MD.h:1210-1222), the box move's dU does not
+     * (MD.cpp:664 drops doBallDPotential's value). */
+    int32_t n_balls;
+    const int32_t *ball;        /* 2 per record: centre, member */
+    const double *ball_const;   /* 2 per record: r0, k */
     /* > 0: the Langevin sigma = sqrt(6 T gamma / dt) is taken once at this temperature and
      * kept when the temperature changes, as the gammaType thermostat of MD.cpp:135-138 does
      * (langevin.h:235-241 computes its per-type sigma on the first call only); 0: sigma

@@ -649,6 +649,19 @@ static double generic_terms(const orc_sys *s, int mode, double *acc, const doubl
             total += (oldP - newP);
         }
     }
+    /* BALL (doBallForce system.h:1937-1971, doBallPotential :2749-2780): harmonicHalfF /
+     * harmonicHalfP (MD.h:1164-1222), active only for r < r0; the box move adds nothing
+     * for them (MD.cpp:664 drops doBallDPotential's value) */
+    for (int j = 0; j < s->nball; j++) {
+        const int32_t f = s->ball[2 * j], g = s->ball[2 * j + 1];
+        const double *c = s->ball_const + 2 * j;
+        double d[3];
+        min_image_diff(s->pos + 3 * (size_t)f, s->pos + 3 * (size_t)g, s->box, d);
+        const double dr2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (dr2 >= c[0] * c[0] || mode == 2) continue;
+        if (mode == 0) harmonic_f(d, acc + 3 * (size_t)f, acc + 3 * (size_t)g, c);
+        else total += harmonic_p(d, c);
+    }
     for (int j = 0; j < s->nbend; j++) {
         const int32_t f = s->bend[3 * j], g = s->bend[3 * j + 1], h = s->bend[3 * j + 2];
         const double *c = s->bend_const + 2 * j;

@@ -37,6 +37,9 @@ typedef struct {
     int32_t nbend;              /* generic BEND records: triples ... */
     const int32_t *bend;        /* nbend*3 */
     const double *bend_const;   /* nbend*2: cos-theta term, k_bend (bendF, MD.h:683-723) */
+    int32_t nball;              /* BALL records: (centre, member) ... */
+    const int32_t *ball;        /* nball*2 */
+    const double *ball_const;   /* nball*2: r0, k (harmonicHalfF, MD.h:1164-1187) */
 } orc_sys;
 
 /* OpenMP thread count for the oracle's parallel loops (1 gives the reference's

@@ -17,7 +17,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-BOND, BEND, CHAIN = 6, 7, 8  # molecule type ids (include/algorithms/molecules.h:14-16)
+BOND, BEND, CHAIN, BALL = 6, 7, 8, 19  # molecule type ids (include/algorithms/molecules.h:14-16,28)
 
 
 def build() -> None:
@@ -51,6 +51,7 @@ class OrcSys(C.Structure):
         ("seed", C.c_uint32),
         ("nbond", C.c_int32), ("bond", C.POINTER(C.c_int32)), ("bond_const", C.POINTER(C.c_double)),
         ("nbend", C.c_int32), ("bend", C.POINTER(C.c_int32)), ("bend_const", C.POINTER(C.c_double)),
+        ("nball", C.c_int32), ("ball", C.POINTER(C.c_int32)), ("ball_const", C.POINTER(C.c_double)),
     ]
 
 
@@ -110,8 +111,8 @@ def _ip(a):
 # ---------------------------------------------------------------------------------------
 # .mpd tokenizer (test infrastructure; the product has its own C++ reader)
 # ---------------------------------------------------------------------------------------
-_NCONST = {6: 2, 7: 2, 8: 4}    # BOND, BEND, CHAIN constants (system.h:1049-1060)
-_NPART = {6: 2, 7: 3, 8: 3}     # indices per record
+_NCONST = {6: 2, 7: 2, 8: 4, 19: 2}    # BOND, BEND, CHAIN, BALL constants (system.h:1049-1110)
+_NPART = {6: 2, 7: 3, 8: 3, 19: 2}     # indices per record
 
 
 def read_mpd(path: str) -> dict:
@@ -182,7 +183,7 @@ class System:
 
     def __init__(self, pos, type_, vel, box, fconst, uconst, ntypes, chains, chain_const,
                  cutoff=2.0, dt=0.02, gamma=1.0, temp=3.0, delta_lxy=0.0, tension=0.0,
-                 seed=0, bonds=None, bond_const=None, bends=None, bend_const=None):
+                 seed=0, bonds=None, bond_const=None, bends=None, bend_const=None, balls=None, ball_const=None):
         self.pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3).copy()
         self.type = np.ascontiguousarray(type_, dtype=np.int32).copy()
         self.vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3).copy()
@@ -198,7 +199,11 @@ class System:
         self.bend = np.ascontiguousarray(bends if bends is not None else [], dtype=np.int32).reshape(-1, 3).copy()
         self.bend_const = np.ascontiguousarray(bend_const if bend_const is not None else [],
                                                dtype=np.float64).reshape(-1, 2).copy()
+        self.ball = np.ascontiguousarray(balls if balls is not None else [], dtype=np.int32).reshape(-1, 2).copy()
+        self.ball_const = np.ascontiguousarray(ball_const if ball_const is not None else [],
+                                               dtype=np.float64).reshape(-1, 2).copy()
         assert len(self.bond_const) == len(self.bond) and len(self.bend_const) == len(self.bend)
+        assert len(self.ball_const) == len(self.ball)
         self.s = OrcSys()
         s = self.s
         s.n = len(self.type)
@@ -212,17 +217,23 @@ class System:
         s.delta_lxy, s.tension, s.seed = delta_lxy, tension, seed & 0xFFFFFFFF
         s.nbond, s.bond, s.bond_const = len(self.bond), _ip(self.bond), _dp(self.bond_const)
         s.nbend, s.bend, s.bend_const = len(self.bend), _ip(self.bend), _dp(self.bend_const)
+        s.nball, s.ball, s.ball_const = len(self.ball), _ip(self.ball), _dp(self.ball_const)
 
     @classmethod
     def from_mpd(cls, path: str) -> "System":
         m = read_mpd(path)
         chains, cc = [], []
-        rec = {BOND: ([], []), BEND: ([], [])}
+        rec = {BOND: ([], []), BEND: ([], []), BALL: ([], [])}
         for mol in m.get("molecule", []):
             if mol["type"] == CHAIN:
                 for r in mol["bonds"]:
                     chains.append(r)
                     cc.append(mol["const"])
+            elif mol["type"] == BALL:
+                # the force takes the centre from the first record (system.h:1944)
+                for r in mol["bonds"]:
+                    rec[BALL][0].append([mol["bonds"][0][0], r[1]])
+                    rec[BALL][1].append(mol["const"])
             elif mol["type"] in rec:
                 for r in mol["bonds"]:
                     rec[mol["type"]][0].append(r)
@@ -235,7 +246,7 @@ class System:
                    gamma=m.get("gamma", 0.0), temp=m["initialTemp"],
                    delta_lxy=m.get("deltaLXY", 0.0), tension=m.get("tension", 0.0),
                    seed=m["seed"], bonds=rec[BOND][0], bond_const=rec[BOND][1],
-                   bends=rec[BEND][0], bend_const=rec[BEND][1])
+                   bends=rec[BEND][0], bend_const=rec[BEND][1], balls=rec[BALL][0], ball_const=rec[BALL][1])
 
     @property
     def box(self):

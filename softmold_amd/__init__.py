@@ -45,6 +45,7 @@ class SmSystem(C.Structure):
         ("seed", C.c_uint64),
         ("n_bonds", C.c_int32), ("bond", C.POINTER(C.c_int32)), ("bond_const", C.POINTER(C.c_double)),
         ("n_bends", C.c_int32), ("bend", C.POINTER(C.c_int32)), ("bend_const", C.POINTER(C.c_double)),
+        ("n_balls", C.c_int32), ("ball", C.POINTER(C.c_int32)), ("ball_const", C.POINTER(C.c_double)),
         ("sigma_temperature", C.c_double),
     ]
 
@@ -208,6 +209,9 @@ class Mpd:
             "bend": arr(s.bend, 3 * s.n_bends, np.int32).reshape(-1, 3) if s.n_bends else np.zeros((0, 3), np.int32),
             "bend_const": (arr(s.bend_const, 2 * s.n_bends, np.float64).reshape(-1, 2) if s.n_bends
                            else np.zeros((0, 2))),
+            "ball": arr(s.ball, 2 * s.n_balls, np.int32).reshape(-1, 2) if s.n_balls else np.zeros((0, 2), np.int32),
+            "ball_const": (arr(s.ball_const, 2 * s.n_balls, np.float64).reshape(-1, 2) if s.n_balls
+                           else np.zeros((0, 2))),
             "sigma_temperature": s.sigma_temperature,
         }
 
@@ -247,6 +251,8 @@ def _pack(d: dict, keep: list) -> SmSystem:
         "bond_const": np.ascontiguousarray(d.get("bond_const", np.zeros((0, 2))), np.float64).reshape(-1, 2),
         "bend": np.ascontiguousarray(d.get("bend", np.zeros((0, 3))), np.int32).reshape(-1, 3),
         "bend_const": np.ascontiguousarray(d.get("bend_const", np.zeros((0, 2))), np.float64).reshape(-1, 2),
+        "ball": np.ascontiguousarray(d.get("ball", np.zeros((0, 2))), np.int32).reshape(-1, 2),
+        "ball_const": np.ascontiguousarray(d.get("ball_const", np.zeros((0, 2))), np.float64).reshape(-1, 2),
     }
     if d.get("acc") is not None:
         k["acc"] = np.ascontiguousarray(d["acc"], np.float64).reshape(-1, 3)
@@ -265,6 +271,7 @@ def _pack(d: dict, keep: list) -> SmSystem:
     s.seed = int(d["seed"]) & 0xFFFFFFFFFFFFFFFF
     s.n_bonds, s.bond, s.bond_const = len(k["bond"]), _ip(k["bond"]), _dp(k["bond_const"])
     s.n_bends, s.bend, s.bend_const = len(k["bend"]), _ip(k["bend"]), _dp(k["bend_const"])
+    s.n_balls, s.ball, s.ball_const = len(k["ball"]), _ip(k["ball"]), _dp(k["ball_const"])
     s.sigma_temperature = float(d.get("sigma_temperature", 0.0))
     return s
 

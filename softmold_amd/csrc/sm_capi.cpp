@@ -483,18 +483,20 @@ void validate(const sm_system *s)
                     unsupported("non-symmetric two-body constant tables are not supported");
     if (s->n_chain_blocks < 0) invalid("negative chain block count");
     if (s->n_chain_blocks > 0 && (!s->chain || !s->chain_const)) invalid("chain blocks without data");
-    if (s->n_bonds < 0 || s->n_bends < 0) invalid("negative BOND / BEND record count");
-    if ((s->n_bonds > 0 && (!s->bond || !s->bond_const)) || (s->n_bends > 0 && (!s->bend || !s->bend_const)))
-        invalid("BOND / BEND records without data");
+    if (s->n_bonds < 0 || s->n_bends < 0 || s->n_balls < 0) invalid("negative BOND / BEND / BALL record count");
+    if ((s->n_bonds > 0 && (!s->bond || !s->bond_const)) || (s->n_bends > 0 && (!s->bend || !s->bend_const)) ||
+        (s->n_balls > 0 && (!s->ball || !s->ball_const)))
+        invalid("BOND / BEND / BALL records without data");
     auto in_range = [&](const int32_t *t, int m) {
         for (int k = 0; k < m; k++)
-            if (t[k] < 0 || t[k] >= n) invalid("a BOND / BEND record names a particle out of range");
+            if (t[k] < 0 || t[k] >= n) invalid("a BOND / BEND / BALL record names a particle out of range");
         for (int k = 0; k < m; k++)
             for (int l = k + 1; l < m; l++)
-                if (t[k] == t[l]) invalid(std::string("a ") + (m == 2 ? "BOND" : "BEND") + " record repeats a particle");
+                if (t[k] == t[l]) invalid("a BOND / BEND / BALL record repeats a particle");
     };
     for (int j = 0; j < s->n_bonds; j++) in_range(s->bond + 2 * j, 2);
     for (int j = 0; j < s->n_bends; j++) in_range(s->bend + 3 * j, 3);
+    for (int j = 0; j < s->n_balls; j++) in_range(s->ball + 2 * j, 2);
 }
 
 // replicas of one context share the interaction constants, cutoff and time step
@@ -581,6 +583,10 @@ sm_ctx *create(const sm_system *s, int nrep)
                 grec.push_back(GenEnt{0, bead(s[r].bond[2 * j]), bead(s[r].bond[2 * j + 1]), -1});
                 gcst.push_back(make_double2(s[r].bond_const[2 * j], s[r].bond_const[2 * j + 1]));
             }
+            for (int j = 0; j < s[r].n_balls; j++) {
+                grec.push_back(GenEnt{2, bead(s[r].ball[2 * j]), bead(s[r].ball[2 * j + 1]), -1});
+                gcst.push_back(make_double2(s[r].ball_const[2 * j], s[r].ball_const[2 * j + 1]));
+            }
             for (int j = 0; j < s[r].n_bends; j++) {
                 grec.push_back(GenEnt{1, bead(s[r].bend[3 * j]), bead(s[r].bend[3 * j + 1]), bead(s[r].bend[3 * j + 2])});
                 gcst.push_back(make_double2(s[r].bend_const[2 * j], s[r].bend_const[2 * j + 1]));
@@ -594,7 +600,7 @@ sm_ctx *create(const sm_system *s, int nrep)
                 out[0] = g.p1;
                 out[1] = g.p2;
                 out[2] = g.p3;
-                return g.kind == 0 ? 2 : 3;
+                return g.kind == 1 ? 3 : 2;
             };
             for (const GenEnt &g : grec) {
                 int mb[3];

@@ -58,8 +58,8 @@ struct DevRep {
     double hist_reject[kHistBins];
 };
 
-// Generic BOND / BEND record as seen from one of its beads (global bead indices):
-// kind 0 = bond (p1, p2), kind 1 = bend (p1, p2, p3).  Each bead lists the records it
+// Generic BOND / BEND / BALL record as seen from one of its beads (global bead indices):
+// kind 0 = bond (p1, p2), kind 1 = bend (p1, p2, p3), kind 2 = ball (centre p1, member p2).  Each bead lists the records it
 // belongs to (CSR: gptr[i] .. gptr[i+1]); the record's two constants sit beside it.
 struct __align__(16) GenEnt {
     int kind;

@@ -236,6 +236,11 @@ __global__ __launch_bounds__(kBlock) void k_bonded_kick(int n, const Bead *__res
             if (g.kind == 0) {
                 f = bond_f(min_image(beads[g.p1], beads[g.p2], L0, L1, L2), K.x, K.y);
                 if (i != g.p1) { f.x = -f.x; f.y = -f.y; f.z = -f.z; }
+            } else if (g.kind == 2) {
+                // harmonicHalfF (MD.h:1164-1187): only inside r0
+                const double3 d = min_image(beads[g.p1], beads[g.p2], L0, L1, L2);
+                f = (d.x * d.x + d.y * d.y + d.z * d.z < K.x * K.x) ? bond_f(d, K.x, K.y) : make_double3(0.0, 0.0, 0.0);
+                if (i != g.p1) { f.x = -f.x; f.y = -f.y; f.z = -f.z; }
             } else {
                 const Bead q1 = beads[g.p1], q2 = beads[g.p2], q3 = beads[g.p3];
                 f = bend_f(min_image(q1, q2, L0, L1, L2), min_image(q2, q3, L0, L1, L2), K.x, K.y,
@@ -415,6 +420,9 @@ __global__ __launch_bounds__(kBlock) void k_gen_energy(int n, GenTables gen, con
                 for (int d = 0; d < 3; d++) da[d] *= S[d];
                 un = harm_p(da, K.x, K.y);
             }
+        } else if (g.kind == 2) {
+            // harmonicHalfP (MD.h:1210-1222) in U; nothing in dU (MD.cpp:664)
+            u = (!delta && da[0] * da[0] + da[1] * da[1] + da[2] * da[2] < K.x * K.x) ? harm_p(da, K.x, K.y) : 0.0;
         } else {
             mi3(beads[g.p2], beads[g.p3], L, db);
             u = bend_p(da, db, K.x, K.y);
@@ -454,7 +462,7 @@ __global__ __launch_bounds__(kBlock) void k_gen_stats(int n, GenTables gen, cons
     const int e1 = gen.gptr[i + 1];
     for (int k = gen.gptr[i]; k < e1; k++) {
         const GenEnt g = gen.gent[k];
-        if (g.p1 != i || (g.kind == 0) != (what == 0)) continue;
+        if (g.p1 != i || g.kind == 2 || (g.kind == 0) != (what == 0)) continue;
         double da[3], db[3];
         mi3_closed(beads[g.p1], beads[g.p2], L, da);
         const double ra = sqrt(da[0] * da[0] + da[1] * da[1] + da[2] * da[2]);

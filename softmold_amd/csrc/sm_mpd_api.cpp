@@ -13,8 +13,8 @@ using namespace smapi;
 
 struct sm_mpd {
     smio::System sys;
-    std::vector<int32_t> chain, bond, bend;
-    std::vector<double> chain_const, bond_const, bend_const;
+    std::vector<int32_t> chain, bond, bend, ball;
+    std::vector<double> chain_const, bond_const, bend_const, ball_const;
 };
 
 namespace {
@@ -115,6 +115,8 @@ int sm_mpd_system(const sm_mpd *mc, sm_system *out)
         m->bond_const.clear();
         m->bend.clear();
         m->bend_const.clear();
+        m->ball.clear();
+        m->ball_const.clear();
         for (const auto &mo : s.mol) {
             if (mo.type == smio::CHAIN) {
                 for (const auto &b : mo.bonds) {
@@ -130,9 +132,20 @@ int sm_mpd_system(const sm_mpd *mc, sm_system *out)
                     if (!bond) idx.push_back(b[2]);
                     cst.insert(cst.end(), mo.constants.begin(), mo.constants.begin() + 2);
                 }
+            } else if (mo.type == smio::BALL) {
+                // doBallForce takes the centre from the first record (system.h:1944) and
+                // doBallPotential from each record (:2760); they agree when all records
+                // share it, the only layout accepted here
+                if (mo.bonds.empty()) continue;
+                const int centre = mo.bonds[0][0];
+                for (const auto &b : mo.bonds) {
+                    if (b[0] != centre) unsupported("BALL records with different centre particles");
+                    m->ball.insert(m->ball.end(), {centre, b[1]});
+                    m->ball_const.insert(m->ball_const.end(), mo.constants.begin(), mo.constants.begin() + 2);
+                }
             } else {
                 unsupported("molecule type " + std::to_string(mo.type) +
-                            " is not implemented (CHAIN, BOND and BEND are)");
+                            " is not implemented (CHAIN, BOND, BEND and BALL are)");
             }
         }
         // MD.cpp:129-143: a positive gamma, else the gammaType thermostat, which applies
@@ -167,6 +180,9 @@ int sm_mpd_system(const sm_mpd *mc, sm_system *out)
         out->n_bends = (int32_t)(m->bend.size() / 3);
         out->bend = m->bend.data();
         out->bend_const = m->bend_const.data();
+        out->n_balls = (int32_t)(m->ball.size() / 2);
+        out->ball = m->ball.data();
+        out->ball_const = m->ball_const.data();
     });
 }
 

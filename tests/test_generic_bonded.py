@@ -55,7 +55,7 @@ def _oracle(d, pos=None):
                     d["n_types"], d["chain"], d["chain_const"], cutoff=d["cutoff"], dt=d["dt"],
                     gamma=d["gamma"], temp=d["temperature"], delta_lxy=d["delta_lxy"], tension=d["tension"],
                     seed=d["seed"], bonds=d.get("bond"), bond_const=d.get("bond_const"), bends=d.get("bend"),
-                    bend_const=d.get("bend_const"))
+                    bend_const=d.get("bend_const"), balls=d.get("ball"), ball_const=d.get("ball_const"))
 
 
 def _deformed(d, seed=7, amp=0.15):
@@ -209,4 +209,99 @@ def test_device_bond_stats_match_oracle():
     pos = s.download()[0]
     want = O.bond_stats(pos, d["box"], d["bond"], d["bend"])
     assert s.bond_stats() == pytest.approx(want, rel=1e-12)
+    s.close()
+
+
+# ---- BALL records ---------------------------------------------------------------------
+
+def _with_ball(d, n_members=40, r0=3.0, k=50.0, seed=5):
+    """d plus one BALL: a head bead as the centre and the nearest other heads as members,
+    so some sit inside r0 (repelled) and some outside (no force)."""
+    rng = np.random.default_rng(seed)
+    heads = np.flatnonzero(d["type"] == 2)
+    c = int(rng.choice(heads))
+    dist = np.linalg.norm(d["pos"][heads] - d["pos"][c], axis=1)
+    members = heads[np.argsort(dist)[1:n_members + 1]]
+    e = dict(d)
+    e["ball"] = np.array([[c, m] for m in members], np.int32)
+    e["ball_const"] = np.tile([r0, k], (len(members), 1))
+    return e
+
+
+def test_oracle_ball_closed_form():
+    """One member at r < r0: U = k (r - r0)^2 / 2 and a push apart of k (r0 - r); a member
+    at r > r0 contributes nothing (harmonicHalfF / harmonicHalfP, MD.h:1164-1222)."""
+    d = dict(sm.Mpd.liposome(5, 300, 3.45).system())
+    d["chain"], d["chain_const"] = np.zeros((0, 3), np.int32), np.zeros((0, 4))
+    pos = d["pos"].copy()
+    pos[1] = pos[0] + np.array([1.0, 0.0, 0.0])
+    pos[2] = pos[0] + np.array([0.0, 3.5, 0.0])
+    d["ball"], d["ball_const"] = np.array([[0, 1], [0, 2]], np.int32), np.array([[2.0, 50.0], [2.0, 50.0]])
+    o = _oracle(d, pos)
+    assert o.chain_potential() == pytest.approx(0.5 * 50.0 * 1.0, rel=1e-12)
+    f = o.chain_force()
+    assert f[0] == pytest.approx([-50.0, 0.0, 0.0], rel=1e-12, abs=1e-12)
+    assert f[1] == pytest.approx([50.0, 0.0, 0.0], rel=1e-12, abs=1e-12)
+    assert np.all(f[2] == 0.0)
+    # the box move's dU leaves BALL terms out (MD.cpp:664)
+    assert o.chain_dpotential([1.01, 0.99, 1.0 / (1.01 * 0.99)]) == 0.0
+
+
+def test_oracle_ball_gradient():
+    d = _with_ball(sm.Mpd.liposome(5, 300, 3.45).system())
+    d["chain"], d["chain_const"] = np.zeros((0, 3), np.int32), np.zeros((0, 4))
+    pos = d["pos"]
+    o = _oracle(d, pos)
+    f = o.chain_force()
+    assert np.abs(f).max() > 1.0                       # some members are inside r0
+    assert np.abs(f.sum(0)).max() <= 1e-10 * np.abs(f).max()
+    h = 1e-6
+    for i in (int(d["ball"][0, 0]), int(d["ball"][0, 1])):
+        for k in range(3):
+            p1, p2 = pos.copy(), pos.copy()
+            p1[i, k] += h
+            p2[i, k] -= h
+            du = (_oracle(d, p1).chain_potential() - _oracle(d, p2).chain_potential()) / (2 * h)
+            assert -du == pytest.approx(f[i, k], rel=1e-5, abs=1e-6)
+
+
+def test_ball_molecule_from_mpd(tmp_path):
+    """A BALL molecule record in a .mpd file: the view takes the first record's centre for
+    every member; records with another centre are refused."""
+    import re
+    m = sm.Mpd.liposome(1, 50, 3.45)
+    m.write(str(tmp_path / "g"))
+    text = (tmp_path / "g.mpd").read_text()
+    # append a BALL as a second molecule after the CHAIN record
+    text2 = re.sub(r"nMolecules 1 ", "nMolecules 2 ", text)
+    i = text2.index("\nmolecule \n")
+    j = text2.index("\n", text2.index(" 0  50  3 ", i)) + 1
+    text2 = text2[:j] + "19\t 3\n 3  50 \n 0  3 \n 0  6 \n 0  9 \n" + text2[j:]
+    (tmp_path / "b.mpd").write_text(text2)
+    d = sm.Mpd.read(str(tmp_path / "b")).system()
+    assert d["ball"].tolist() == [[0, 3], [0, 6], [0, 9]]
+    assert d["ball_const"].tolist() == [[3.0, 50.0]] * 3
+    (tmp_path / "c.mpd").write_text(text2.replace(" 0  6 \n", " 1  6 \n"))
+    with pytest.raises(sm.SmError, match="centre"):
+        sm.Mpd.read(str(tmp_path / "c")).system()
+
+
+@pytest.mark.gpu
+def test_device_ball_matches_oracle():
+    if not gpu_available():
+        pytest.skip("no GPU")
+    d = _with_ball(sm.Mpd.liposome(9, 1500, 3.45).system())
+    s = sm.Solver.from_system(d)
+    _, fc = s.compute_forces()
+    o = _oracle(d)
+    want = o.chain_force()
+    err = np.linalg.norm(fc - want, axis=1)
+    scale = np.maximum(np.linalg.norm(want, axis=1), 0.05 * np.sqrt((want ** 2).sum(1).mean()))
+    assert (err / scale).max() <= 1e-5
+    U = s.measure()
+    assert U[1] == pytest.approx(o.chain_potential(), rel=1e-10, abs=1e-9)
+    sc = np.array([1.002, 0.999, 1.0 / (1.002 * 0.999)])
+    assert s.potential_delta(sc)[1] == pytest.approx(o.chain_dpotential(sc), rel=1e-9, abs=1e-9)
+    st = s.bond_stats()
+    assert np.isnan(st[0]) and list(st[1:]) == [0.0, 0.0, 0.0]     # BALL is not a bond for lBond_
     s.close()
