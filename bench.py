@@ -18,7 +18,8 @@ Prints one JSON line: particle-steps/s, the pair kernel's roofline record (algor
 bytes per launch over its HIP-event-timed average duration), the batched sweep, and the
 CPU baseline (the reference MD binary built from source in oracle/_ref, or the FP64
 oracle port when it is absent), timed on this host on a bounded sample of the plain
-workload.
+workload.  The same CPU leg reports the FP32-vs-FP64 error of the workload at step 0
+(`fp64_check`), with the FP64 oracle as the checker.
 """
 from __future__ import annotations
 
@@ -115,6 +116,34 @@ def cpu_baseline(n_beads: int) -> dict:
                 "sample": f"FP64 oracle restatement of MD.cpp, {steps} steps of the same vesicle"}
     finally:
         shutil.rmtree(work, ignore_errors=True)
+
+
+def fp64_check(d: dict, cores: int) -> dict:
+    """The FP32-vs-FP64 error of this workload at step 0, with the FP64 oracle as the
+    checker: conservative forces (pair + CHAIN) per particle, relative to
+    max(|F_i|, 0.05 rms|F|), and in L2; |dU|/|U| and |dK|/K of the device's FP64 sums."""
+    import softmold_amd as sm
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    O.set_threads(cores)
+    s = sm.Solver.from_system(d)
+    fp, fc = s.compute_forces()
+    upair, uchain, kin = s.measure()
+    s.close()
+    o = O.System(d["pos"], d["type"], d["vel"], d["box"], d["fconst"], d["uconst"], d["n_types"], d["chain"],
+                 d["chain_const"], cutoff=d["cutoff"])
+    want = o.pair_force() + o.chain_force()
+    got = fp + fc
+    err = np.linalg.norm(got - want, axis=1)
+    scale = np.maximum(np.linalg.norm(want, axis=1), 0.05 * np.sqrt((want ** 2).sum(1).mean()))
+    u_ref = o.pair_potential() + o.chain_potential()
+    k_ref = o.kinetic()
+    return {"force_max_rel": float((err / scale).max()),
+            "force_l2_rel": float(np.linalg.norm(got - want) / np.linalg.norm(want)),
+            "u_rel": float(abs(upair + uchain - u_ref) / abs(u_ref)),
+            "k_rel": float(abs(kin - k_ref) / k_ref),
+            "checker": "FP64 oracle restatement of MD.cpp, step 0 of the same vesicle"}
 
 
 def bench_batched(steps: int, warmup: int, seq_steps: int) -> dict:
@@ -291,6 +320,12 @@ def main():
         except Exception as e:  # the GPU number stands on its own
             log("cpu baseline failed:", e)
             out["cpu_baseline"] = None
+        try:
+            cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+            out["fp64_check"] = fp64_check(d, int(os.environ.get("OMP_NUM_THREADS", cores)))
+        except Exception as e:
+            log("fp64 check failed:", e)
+            out["fp64_check"] = None
     print(json.dumps(out), flush=True)
 
 
