@@ -44,6 +44,7 @@ SEED, NLIPIDS, DENSITY = 1234, 80000, 3.45
 # HBM bytes per pair_force launch on this workload, from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE runs scaled by the same counters on a copy of known size (bench.py --calib-copy)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pair_force_vesicle240k_traffic.json")
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "pair_force_vesicle240k_counters.json")
 
 
 def log(*a):
@@ -214,6 +215,17 @@ def calib_copy():
     print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "copies": 5}), flush=True)
 
 
+def valu_issue() -> float | None:
+    """VALU issue fraction of the pair kernel on this workload, from its SQ counters
+    (profiles/pair_force_vesicle240k_counters.json): VALU wave-instructions over what the
+    chip can issue in the kernel's active cycles -- the bound this kernel is held to."""
+    try:
+        with open(COUNTERS_FILE) as f:
+            return float(json.load(f)["valu_issue_frac"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def traffic() -> float | None:
     try:
         with open(TRAFFIC_FILE) as f:
@@ -309,7 +321,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic(),
                      "kernel": "pair_force", "kernel_avg_us": pair_ms * 1e3,
                      "algorithmic_bytes_per_launch": algo_bytes,
-                     "valu_tflops": tflops, "valu_frac_fp32": tflops / VALU_FP32_PEAK_TFLOPS},
+                     "valu_tflops": tflops, "valu_frac_fp32": tflops / VALU_FP32_PEAK_TFLOPS,
+                     "valu_issue_frac": valu_issue()},
     }
     s.close()
     if not args.no_batched:
