@@ -745,8 +745,8 @@ sm_ctx *create(const sm_system *s, int nrep)
         }
         // sorted slots per wave in the pair kernel: short chunks keep small systems filling
         // the chip, long ones cut the per-cell overhead of large ones (measured on the
-        // 15k, 240k and 3M-bead systems; every chunk gives the same forces)
-        c->chunk = n >= 1000000 ? 32 : (n >= 100000 ? 16 : 8);
+        // 12k, 15k, 240k and 3M-bead systems; every chunk gives the same forces)
+        c->chunk = n >= 1000000 ? 16 : (n >= 50000 ? 8 : 4);
         c->ensure_capacity(0);
         HIPCHK(hipStreamSynchronize(st));
     } catch (...) {
@@ -1260,7 +1260,7 @@ int sm_autotune(sm_ctx *c, int32_t steps, double *best_ms_per_step)
             // graph replay (the same kernels on the same stream)
             struct Cand { int chunk, variant, graph; };
             std::vector<Cand> cands;
-            for (int ch : {4, 8, 16, 32, 64}) cands.push_back({ch, 0, 1});
+            for (int ch : {2, 4, 8, 16, 32, 64}) cands.push_back({ch, 0, 1});
             double best = 1e300;
             Cand bestc{c->chunk, c->pair_variant, c->use_graph ? 1 : 0};
             auto trial = [&](const Cand &k) {

@@ -17,9 +17,9 @@
 //      tpp-th candidate; partial results meet in LDS and are combined in lane order.
 // Force, one phase (M_FORCE1, the default): candidates are staged as FP64 imaged
 // positions, and each staged candidate is entered into the list of each half of the home
-// cell (x below / above the cell centre) whose box it comes within rc of; a home particle
-// walks the list of its half only (61 % of the 27-cell candidates on average, no pair
-// within rc is skipped).  A lane takes the FP64 difference, applies the exact FP64 test
+// cell (x below / above the cell centre) whose box it comes within its type's reach of (rc,
+// or r_m for a type whose every pair has zero force and potential beyond r_m); a home
+// particle walks the list of its half only, and no pair with a nonzero term is skipped.  A lane takes the FP64 difference, applies the exact FP64 test
 // r^2 < rc^2 (no contraction, as MD.h:798-810) and under that branch evaluates the force
 // law in FP32 with a Newton-corrected r.  Force, two phase (M_FORCE, kept as a tuning variant):
 // candidates are staged a second time as FP32 positions relative to the home cell
@@ -36,7 +36,7 @@ namespace smdev {
 
 namespace {
 
-constexpr int kWaves = 4;
+constexpr int kWaves = 1;   // waves per workgroup: one, so a finished wave frees its slot at once
 constexpr int kList = 32;      // candidates a lane screens before evaluating its hits
 
 enum : int { M_FORCE = 0, M_U = 1, M_DU = 2, M_COUNT = 3, M_LIST = 4, M_FORCE1 = 5 };
@@ -225,10 +225,35 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
         for (int t = threadIdx.x; t < 6 * nT * nT; t += blockDim.x) sCd[t] = uconst[t];
     }
     __syncthreads();
+    // Culling radius per candidate type: a type pair whose outer coefficients are zero
+    // (Umin = 0 in laradjiRevalee.h:97-110, every pair but tail-tail in the lipid models)
+    // has zero force and potential beyond r_m, so a candidate of type t matters only within
+    // the largest such reach over the home types: r_c where some pair with t has an outer
+    // branch, else r_m.  Candidates culled by it contribute exactly zero.
+    __shared__ float sReach2[kMaxTypes];
+    if (kCull) {
+        if ((int)threadIdx.x < nT) {
+            const int t = threadIdx.x;
+            double r = 0.0;
+            for (int h = 0; h < nT; h++) {
+                const int row = 6 * (h * nT + t);
+                double rm, rc, c4, c5;
+                if (kF) {
+                    const float *c8 = reinterpret_cast<const float *>(sC8) + 8 * (h * nT + t);
+                    rm = c8[0]; rc = c8[3]; c4 = c8[4]; c5 = c8[5];
+                } else {
+                    rm = sCd[row]; rc = sCd[row + 3]; c4 = sCd[row + 4]; c5 = sCd[row + 5];
+                }
+                r = fmax(r, (c4 != 0.0 || c5 != 0.0) ? rc : rm);
+            }
+            r = fmin(r, sqrt(cut2)) + 1e-4;
+            sReach2[t] = (float)(r * r);
+        }
+        __syncthreads();
+    }
 
     const int lane = threadIdx.x & (kWave - 1);
     const unsigned long long lanes_lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-    const float reach2 = cut2pre;   // one-phase kernel: (rc + margin)^2 for the half-cell culling
     const int wave = threadIdx.x / kWave;
     const long long a0 = ((long long)xcd_block(blockIdx.x, nblocks) * kWaves + wave) * chunk;
     const int a = (int)min(a0, (long long)n);
@@ -257,7 +282,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
             // energy gathers: a candidate farther than (rc + margin) / min(1, scale) from the
             // home half-cell is beyond rc both before and after the trial scaling
             const double smin = fmin(1.0, fmin(ax, fmin(ay, az)));
-            const float reachE = MODE == M_DU ? (float)((double)reach2 / (smin * smin)) : reach2;
+            const float reachS = MODE == M_DU ? (float)(1.0 / (smin * smin)) : 1.f;   // dU: widened by the trial scale
             const uint32_t kbase = (uint32_t)R.cellbase;
             const uint32_t k = kg - kbase;
             const int kx = (int)(k % (uint32_t)nx);
@@ -357,7 +382,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                             const float base = gy * gy + gz * gz;
                             const float g0 = fmaxf(fmaxf(rx, -hcx - rx), 0.f);
                             const float g1 = fmaxf(fmaxf(-rx, rx - hcx), 0.f);
-                            const float r2max = kE ? reachE : reach2;
+                            const float r2max = sReach2[qb[it].type] * reachS;
                             reach[it] = (base + g0 * g0 < r2max ? 1 : 0) | (base + g1 * g1 < r2max ? 2 : 0);
                         }
                         // forces stage imaged positions; gathers the stored ones (the image
@@ -557,11 +582,9 @@ void launch_cells(hipStream_t st, const Params &P, int chunk, const Bead *sbeads
     // With at least 3 cells per axis, a cell is shorter than 4/3 of the cutoff.
     const double csmax = P.cutoff * 4.0 / 3.0;
     const double slack = 64.0 * P.cutoff * 1.5 * csmax * (1.0 / 8388608.0) + 1e-6 * cut2;
-    // culled modes (one-phase force, energy gathers): the half-cell culling radius, rc plus a
-    // margin far above the FP32 rounding of cell-relative coordinates (|r| < 3 cells) and the
-    // cell-assignment rounding; the dU gather widens it by its trial scale in the kernel
-    const bool culled = MODE == M_FORCE1 || MODE == M_U || MODE == M_DU;
-    const float cut2pre = culled ? (float)((P.cutoff + 1e-4) * (P.cutoff + 1e-4)) : (float)(cut2 + slack);
+    // the two-phase kernel's FP32 screening radius (the culled modes take theirs per
+    // candidate type from the constant tables, in the kernel)
+    const float cut2pre = (float)(cut2 + slack);
     hipLaunchKernelGGL(k_cells<MODE>, dim3(nb), dim3(kWaves * kWave), 0, st, P.n, P.nT, chunk, nb, cut2, cut2pre,
                        sbeads, skey, sidx, cstart, cend, reps, fconst, uconst, out);
 }

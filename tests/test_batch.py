@@ -159,7 +159,7 @@ def test_autotune_restores_state_and_keeps_results():
     for x, y in zip(before, after):
         assert np.array_equal(x, y)
     chunk, variant, graph = a.tune_get("pair_chunk"), a.tune_get("pair_variant"), a.tune_get("graph")
-    assert chunk in (4, 8, 16, 32, 64) and variant in (0, 1) and graph in (0, 1)
+    assert chunk in (2, 4, 8, 16, 32, 64) and variant in (0, 1) and graph in (0, 1)
     a.run(11, 30)
     b = sm.Solver.batch(systems)
     b.init_forces()
