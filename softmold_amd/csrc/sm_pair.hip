@@ -10,9 +10,10 @@
 //   1. lanes 0..26 look up the 27 stencil cells in the dense cell table (validated
 //      against the sorted keys, so the table is never cleared);
 //   2. a wave scan of the 27 counts lays the candidates out contiguously and every lane
-//      stages its share into LDS (the stencil cell of an item by a binary search over the
-//      scan's lanes), so all global loads of a pass are in flight together; a pass holds
-//      kMaxC candidates and runs only the 64-slot rounds it fills;
+//      stages its share into LDS (the stencil cell of an item from a per-pass LDS map of
+//      the cells' first slots and a DPP max-scan; its image shift and sorted slot from
+//      per-cell LDS tables), so all global loads of a pass are in flight together; a
+//      pass holds kMaxC candidates and runs only the 64-slot rounds it fills;
 //   3. tpp = 64/H lanes serve each of the H home particles, each lane walking every
 //      tpp-th candidate; partial results meet in LDS and are combined in lane order.
 // Force, one phase (M_FORCE1, the default): candidates are staged as FP64 imaged
@@ -55,6 +56,26 @@ __device__ inline int wave_incl_scan(int v, int lane)
     }
     return v;
 }
+
+// Inclusive prefix maximum over the wave of non-negative values, in DPP steps: shifts by
+// 1, 2, 4, 8 within each 16-lane row, then row 0's and row 2's last lanes into rows 1 and
+// 3, then lane 31 into rows 2 and 3.  Lanes a step does not reach keep their value.
+__device__ inline int wave_max_scan(int v)
+{
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true));   // row_shr:1
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true));   // row_shr:2
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true));   // row_shr:4
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true));   // row_shr:8
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return v;
+}
+
+// Image shift of one stencil cell: the box length added to a candidate's coordinates
+// where the stencil crosses the periodic boundary.
+struct __align__(16) Shift {
+    double x, y, z, pad;
+};
 
 __device__ inline void wave_lds_sync()
 {
@@ -134,11 +155,11 @@ __device__ constexpr uint32_t forward_mask()
 // A staged candidate read as two 16-byte LDS loads, so its type arrives with its position.
 __device__ inline Cand load_cand(const Cand *p)
 {
-    const uint4 *q = reinterpret_cast<const uint4 *>(p);
-    const uint4 a = q[0], b = q[1];
+    const double2 a = *reinterpret_cast<const double2 *>(p);
+    const uint4 b = reinterpret_cast<const uint4 *>(p)[1];
     Cand c;
-    c.x = __hiloint2double((int)a.y, (int)a.x);
-    c.y = __hiloint2double((int)a.w, (int)a.z);
+    c.x = a.x;
+    c.y = a.y;
     c.z = __hiloint2double((int)b.y, (int)b.x);
     c.t = (int)b.z;
     c.c = (int)b.w;
@@ -207,16 +228,27 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     static_assert(kMaxC % kWave == 0, "staging rounds cover whole waves");
     __shared__ float4 sC8[kF ? 2 * kMaxTypes * kMaxTypes : 1];   // rows padded to 8 floats
     __shared__ double sCd[kF ? 1 : 6 * kMaxTypes * kMaxTypes];
-    __shared__ Cand sCand[kWaves][kMaxC];
+    // staged candidates; after a chunk's passes the same memory holds the per-lane partial
+    // results that are combined per home particle (LDS per workgroup bounds the resident
+    // waves, so the reduction takes no space of its own)
+    union WaveBuf {
+        Cand cand[kMaxC];
+        float f[3][kWave];
+        double d[kWave];
+        int i[kWave];
+    };
+    __shared__ WaveBuf sBuf[kWaves];
     __shared__ float4 sRel[k2P ? kWaves : 1][k2P ? kMaxC : 1];
     __shared__ uint8_t sList[k2P ? kWaves : 1][k2P ? kList : 1][kWave];   // per-lane hit lists, lane-minor
     // per half-cell candidate lists, padded so the force loop's reads one trip ahead (at most
     // 3 x 64 entries past the end) stay inside the row
     constexpr int kHalfRow = kMaxC + 3 * kWave;
     __shared__ uint8_t sHalf[kCull ? kWaves : 1][2][kCull ? kHalfRow : 1];
-    __shared__ float redf[kF ? kWaves : 1][3][kWave];
-    __shared__ double redd[kF ? 1 : kWaves][kWave];
-    __shared__ int redi[kWaves][kWave];
+    // per stencil cell: image shift, and slot in the sorted order minus slot in the pass
+    // layout; per pass: the stencil cell at each cell's first staged slot (0 elsewhere)
+    __shared__ Shift sShift[kWaves][27];
+    __shared__ int sDelta[kWaves][27];
+    __shared__ uint32_t sMap[kWaves][kMaxC / 4];
 
     if (kF) {
         float *c8 = reinterpret_cast<float *>(sC8);
@@ -259,7 +291,8 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     const int a = (int)min(a0, (long long)n);
     const int aend = min(a + chunk, n);
     constexpr uint32_t fwd = forward_mask();
-    Cand *Cw = sCand[wave];
+    WaveBuf &Bw = sBuf[wave];
+    Cand *Cw = Bw.cand;
     float4 *Rw = sRel[k2P ? wave : 0];
 
     for (int base = a; base < aend; base += kWave) {
@@ -296,6 +329,14 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
             // shell
             constexpr bool kOwnedOnly = MODE == M_U || MODE == M_DU;
             int nst = 0, ncnt = 0;
+            Shift shl;
+            shl.x = shl.y = shl.z = shl.pad = 0.0;
+            if (lane < 27) {
+                const int X0 = kx + lane % 3 - 1, Y0 = ky + (lane / 3) % 3 - 1, Z0 = kz + lane / 9 - 1;
+                shl.x = X0 < 0 ? -Lx : (X0 >= nx ? Lx : 0.0);
+                shl.y = Y0 < 0 ? -Ly : (Y0 >= ny ? Ly : 0.0);
+                shl.z = Z0 < 0 ? -Lz : (Z0 >= nz ? Lz : 0.0);
+            }
             if (lane < 27 && (!kOwnedOnly || lane == 13 || ((fwd >> lane) & 1u))) {
                 int X0 = kx + lane % 3 - 1, Y0 = ky + (lane / 3) % 3 - 1, Z0 = kz + lane / 9 - 1;
                 X0 = X0 < 0 ? X0 + nx : (X0 >= nx ? X0 - nx : X0);
@@ -315,6 +356,12 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
             const int total = __builtin_amdgcn_readlane(incl, 26);
             const int selfOff = __builtin_amdgcn_readlane(excl, 13);
             const int nh = e0 - s0;
+            Shift *ShW = sShift[wave];
+            if (lane < 27) {
+                ShW[lane] = shl;
+                sDelta[wave][lane] = nst - excl;
+            }
+            uint8_t *mapW = reinterpret_cast<uint8_t *>(sMap[wave]);
 
             for (int h0 = 0; h0 < nh; h0 += kWave) {
                 const int H = min(kWave, nh - h0);
@@ -339,23 +386,24 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                     // past it are skipped, so a short last pass costs what it holds
                     const int iters = (ce - cb + kWave - 1) / kWave;
                     // stage [cb, ce): addresses first, then all loads, then the LDS stores
+                    // map of the pass: every stencil cell with slots in [cb, ce) marks its
+                    // first one with its index (the cell holding slot cb marks position 0),
+                    // and a max-scan over the positions gives each slot its cell
+                    if (lane < kMaxC / 4) sMap[wave][lane] = 0u;
+                    if (lane < 27 && ncnt > 0 && incl > cb && excl < ce) mapW[max(excl, cb) - cb] = (uint8_t)lane;
+                    wave_lds_sync();
                     int src[kStageIters], cel[kStageIters];
+                    int carry = 0;
 #pragma unroll
                     for (int it = 0; it < kStageIters; it++) {
                         src[it] = -1;
                         cel[it] = 0;
                         if (it >= iters) continue;
                         const int t = cb + lane + it * kWave;
-                        // the stencil cell holding slot t: the number of cells whose
-                        // inclusive end is <= t, by binary search over lanes 0..31 of the
-                        // scan (lanes 27..31 hold `total`, above every t)
-                        int c = 0;
-#pragma unroll
-                        for (int step = 16; step > 0; step >>= 1)
-                            c += (__shfl(incl, c + step - 1) <= t) ? step : 0;
-                        const int stc = __shfl(nst, c), exc = __shfl(excl, c);
+                        const int c = max(wave_max_scan(t < ce ? (int)mapW[t - cb] : 0), carry);
+                        carry = __builtin_amdgcn_readlane(c, kWave - 1);
                         cel[it] = c;
-                        src[it] = (t < ce) ? stc + (t - exc) : -1;
+                        src[it] = (t < ce) ? t + sDelta[wave][c] : -1;
                     }
                     Bead qb[kStageIters];
 #pragma unroll
@@ -368,10 +416,10 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         if (it >= iters || src[it] < 0) continue;
                         const int c = cel[it];
                         Cand cd;
-                        const int X0 = kx + c % 3 - 1, Y0 = ky + (c / 3) % 3 - 1, Z0 = kz + c / 9 - 1;
-                        const double ix = qb[it].x + (X0 < 0 ? -Lx : (X0 >= nx ? Lx : 0.0));
-                        const double iy = qb[it].y + (Y0 < 0 ? -Ly : (Y0 >= ny ? Ly : 0.0));
-                        const double iz = qb[it].z + (Z0 < 0 ? -Lz : (Z0 >= nz ? Lz : 0.0));
+                        const Shift sh = ShW[c];
+                        const double ix = qb[it].x + sh.x;
+                        const double iy = qb[it].y + sh.y;
+                        const double iz = qb[it].z + sh.z;
                         if (k2P)
                             Rw[lane + it * kWave] = make_float4((float)(ix - ox0), (float)(iy - oy0), (float)(iz - oz0), 0.f);
                         if (kCull) {
@@ -477,10 +525,10 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                             const Cand q = Cw[i];
                             const int c = q.c & 31;
                             if (j == self || (c == 13 && j < self)) continue;
-                            const int X0 = kx + c % 3 - 1, Y0 = ky + (c / 3) % 3 - 1, Z0 = kz + c / 9 - 1;
-                            const double px = q.x + (X0 < 0 ? -Lx : (X0 >= nx ? Lx : 0.0));
-                            const double py = q.y + (Y0 < 0 ? -Ly : (Y0 >= ny ? Ly : 0.0));
-                            const double pz = q.z + (Z0 < 0 ? -Lz : (Z0 >= nz ? Lz : 0.0));
+                            const Shift sh = ShW[c];
+                            const double px = q.x + sh.x;
+                            const double py = q.y + sh.y;
+                            const double pz = q.z + sh.z;
                             const double *C = Ch + 6 * q.t;
                             const double r2 = sq3(hb.x - px, hb.y - py, hb.z - pz);
                             if (MODE == M_U) {
@@ -499,10 +547,8 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                             if (j == self) continue;
                             const Cand q = Cw[j - cb];
                             const int c = q.c & 31;
-                            const int X0 = kx + c % 3 - 1, Y0 = ky + (c / 3) % 3 - 1, Z0 = kz + c / 9 - 1;
-                            const double sx = X0 < 0 ? -Lx : (X0 >= nx ? Lx : 0.0);
-                            const double sy = Y0 < 0 ? -Ly : (Y0 >= ny ? Ly : 0.0);
-                            const double sz = Z0 < 0 ? -Lz : (Z0 >= nz ? Lz : 0.0);
+                            const Shift sh = ShW[c];
+                            const double sx = sh.x, sy = sh.y, sz = sh.z;
                             const bool own = (c == 13) || ((fwd >> c) & 1u);
                             double ox, oy, oz, px, py, pz;
                             if (own) {
@@ -531,36 +577,36 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                     wave_lds_sync();
                 }
                 if (kF) {
-                    redf[kF ? wave : 0][0][lane] = fx;
-                    redf[kF ? wave : 0][1][lane] = fy;
-                    redf[kF ? wave : 0][2][lane] = fz;
+                    Bw.f[0][lane] = fx;
+                    Bw.f[1][lane] = fy;
+                    Bw.f[2][lane] = fz;
                     wave_lds_sync();
                     if (act && sub == 0) {
                         float sx = 0.f, sy = 0.f, sz = 0.f;
                         for (int t = 0; t < tpp; t++) {
-                            sx += redf[kF ? wave : 0][0][lane + t];
-                            sy += redf[kF ? wave : 0][1][lane + t];
-                            sz += redf[kF ? wave : 0][2][lane + t];
+                            sx += Bw.f[0][lane + t];
+                            sy += Bw.f[1][lane + t];
+                            sz += Bw.f[2][lane + t];
                         }
                         out.force[me] = make_float4(sx, sy, sz, 0.f);
                     }
                     wave_lds_sync();
                 } else if (MODE == M_COUNT) {
-                    redi[wave][lane] = cnt;
+                    Bw.i[lane] = cnt;
                     wave_lds_sync();
                     if (act && sub == 0) {
                         int c = 0;
-                        for (int t = 0; t < tpp; t++) c += redi[wave][lane + t];
+                        for (int t = 0; t < tpp; t++) c += Bw.i[lane + t];
                         out.counts[me] = c;
                     }
                     wave_lds_sync();
                 } else if (MODE == M_U || MODE == M_DU) {
                     // the home particle's share, its lanes combined in lane order
-                    redd[kF ? 0 : wave][lane] = accu;
+                    Bw.d[lane] = accu;
                     wave_lds_sync();
                     if (act && sub == 0) {
                         double e = 0.0;
-                        for (int t = 0; t < tpp; t++) e += redd[kF ? 0 : wave][lane + t];
+                        for (int t = 0; t < tpp; t++) e += Bw.d[lane + t];
                         out.energy[s0 + h0 + hi] = e;
                     }
                     wave_lds_sync();
