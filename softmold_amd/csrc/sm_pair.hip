@@ -17,10 +17,11 @@
 //   3. tpp = 64/H lanes serve each of the H home particles, each lane walking every
 //      tpp-th candidate; partial results meet in LDS and are combined in lane order.
 // Force, one phase (M_FORCE1, the default): candidates are staged as FP64 imaged
-// positions, and each staged candidate is entered into the list of each half of the home
-// cell (x below / above the cell centre) whose box it comes within its type's reach of (rc,
-// or r_m for a type whose every pair has zero force and potential beyond r_m); a home
-// particle walks the list of its half only, and no pair with a nonzero term is skipped.  A lane takes the FP64 difference, applies the exact FP64 test
+// positions, and each staged candidate is entered into the list of each quarter of the home
+// cell (x below / above the cell centre times y below / above) whose box it comes within
+// its type's reach of (rc, or r_m for a type whose every pair has zero force and potential
+// beyond r_m); a home particle walks the list of its quarter only, and no pair with a
+// nonzero term is skipped.  A lane takes the FP64 difference, applies the exact FP64 test
 // r^2 < rc^2 (no contraction, as MD.h:798-810) and under that branch evaluates the force
 // law in FP32 with a Newton-corrected r.  Force, two phase (M_FORCE, kept as a tuning variant):
 // candidates are staged a second time as FP32 positions relative to the home cell
@@ -30,7 +31,7 @@
 // difference taken in the reference's direction -- the particle of the cell that owns
 // the pair in the 13-cell half shell (cellOpt.h:715) minus the imaged other.  The energy
 // gathers (U, dU) stage only the home cell and its forward half shell and use the same
-// half-cell culling, with the reach radius widened by the trial scale for dU.
+// quarter-cell culling, with the reach radius widened by the trial scale for dU.
 #include "sm_internal.hpp"
 
 namespace smdev {
@@ -222,8 +223,11 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     constexpr bool k2P = MODE == M_FORCE;
     constexpr bool kF1 = MODE == M_FORCE1;
     constexpr bool kE = MODE == M_U || MODE == M_DU;   // FP64 energy gathers
-    constexpr bool kCull = kF1 || kE;                  // candidates culled per half of the home cell
-    constexpr int kMaxC = k2P ? 128 : 192;   // staged candidates per pass (192 measured faster than 256)
+    constexpr bool kCull = kF1 || kE;                  // candidates culled per quarter of the home cell
+    // staged candidates per pass: the force kernel measured fastest at 192; the two-phase
+    // force and the FP64 energy gathers at 128 (their larger constant tables and registers
+    // leave less LDS per resident wave)
+    constexpr int kMaxC = (k2P || kE) ? 128 : 192;
     constexpr int kStageIters = kMaxC / kWave;
     static_assert(kMaxC % kWave == 0, "staging rounds cover whole waves");
     __shared__ float4 sC8[kF ? 2 * kMaxTypes * kMaxTypes : 1];   // rows padded to 8 floats
@@ -240,15 +244,18 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     __shared__ WaveBuf sBuf[kWaves];
     __shared__ float4 sRel[k2P ? kWaves : 1][k2P ? kMaxC : 1];
     __shared__ uint8_t sList[k2P ? kWaves : 1][k2P ? kList : 1][kWave];   // per-lane hit lists, lane-minor
-    // per half-cell candidate lists, padded so the force loop's reads one trip ahead (at most
-    // 3 x 64 entries past the end) stay inside the row
-    constexpr int kHalfRow = kMaxC + 3 * kWave;
-    __shared__ uint8_t sHalf[kCull ? kWaves : 1][2][kCull ? kHalfRow : 1];
     // per stencil cell: image shift, and slot in the sorted order minus slot in the pass
-    // layout; per pass: the stencil cell at each cell's first staged slot (0 elsewhere)
+    // layout
     __shared__ Shift sShift[kWaves][27];
     __shared__ int sDelta[kWaves][27];
-    __shared__ uint32_t sMap[kWaves][kMaxC / 4];
+    // per pass: the stencil cell at each cell's first staged slot (0 elsewhere), then, in
+    // the same memory once the slots are staged, the candidate lists of the four quarters
+    // of the home cell packed one after another (list q at q * kMaxC) and padded so the
+    // force loop's reads one trip ahead (at most 3 x 64 entries past the end of a list)
+    // stay inside the array
+    constexpr int kListBytes = kCull ? 4 * kMaxC + 3 * kWave : kMaxC;
+    static_assert(kListBytes % 4 == 0, "the map is cleared in 32-bit words");
+    __shared__ uint32_t sLists[kWaves][kListBytes / 4];
 
     if (kF) {
         float *c8 = reinterpret_cast<float *>(sC8);
@@ -313,7 +320,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
             const float hcx = (float)(0.5 * csx), hcy = (float)(0.5 * csy), hcz = (float)(0.5 * csz);
             const double ax = R.scale[0], ay = R.scale[1], az = R.scale[2];
             // energy gathers: a candidate farther than (rc + margin) / min(1, scale) from the
-            // home half-cell is beyond rc both before and after the trial scaling
+            // home quarter-cell is beyond rc both before and after the trial scaling
             const double smin = fmin(1.0, fmin(ax, fmin(ay, az)));
             const float reachS = MODE == M_DU ? (float)(1.0 / (smin * smin)) : 1.f;   // dU: widened by the trial scale
             const uint32_t kbase = (uint32_t)R.cellbase;
@@ -361,7 +368,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                 ShW[lane] = shl;
                 sDelta[wave][lane] = nst - excl;
             }
-            uint8_t *mapW = reinterpret_cast<uint8_t *>(sMap[wave]);
+            uint8_t *mapW = reinterpret_cast<uint8_t *>(sLists[wave]);   // map, then lists
 
             for (int h0 = 0; h0 < nh; h0 += kWave) {
                 const int H = min(kWave, nh - h0);
@@ -389,7 +396,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                     // map of the pass: every stencil cell with slots in [cb, ce) marks its
                     // first one with its index (the cell holding slot cb marks position 0),
                     // and a max-scan over the positions gives each slot its cell
-                    if (lane < kMaxC / 4) sMap[wave][lane] = 0u;
+                    if (lane < kMaxC / 4) sLists[wave][lane] = 0u;
                     if (lane < 27 && ncnt > 0 && incl > cb && excl < ce) mapW[max(excl, cb) - cb] = (uint8_t)lane;
                     wave_lds_sync();
                     int src[kStageIters], cel[kStageIters];
@@ -423,15 +430,19 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         if (k2P)
                             Rw[lane + it * kWave] = make_float4((float)(ix - ox0), (float)(iy - oy0), (float)(iz - oz0), 0.f);
                         if (kCull) {
-                            // which half of the home cell (x < centre, x >= centre) the
-                            // candidate can reach: box distance below the reach radius
+                            // which quarters of the home cell (x below / above the centre,
+                            // times y below / above) the candidate can reach: box distance
+                            // below the reach radius; bit qx + 2 qy
                             const float rx = (float)(ix - ox0), ry = (float)(iy - oy0), rz = (float)(iz - oz0);
-                            const float gy = fmaxf(fabsf(ry) - hcy, 0.f), gz = fmaxf(fabsf(rz) - hcz, 0.f);
-                            const float base = gy * gy + gz * gz;
-                            const float g0 = fmaxf(fmaxf(rx, -hcx - rx), 0.f);
-                            const float g1 = fmaxf(fmaxf(-rx, rx - hcx), 0.f);
+                            const float gz = fmaxf(fabsf(rz) - hcz, 0.f);
+                            const float gx0 = fmaxf(fmaxf(rx, -hcx - rx), 0.f), gx1 = fmaxf(fmaxf(-rx, rx - hcx), 0.f);
+                            const float gy0 = fmaxf(fmaxf(ry, -hcy - ry), 0.f), gy1 = fmaxf(fmaxf(-ry, ry - hcy), 0.f);
+                            const float z2 = gz * gz;
+                            const float a0 = fmaf(gx0, gx0, z2), a1 = fmaf(gx1, gx1, z2);
+                            const float b0 = gy0 * gy0, b1 = gy1 * gy1;
                             const float r2max = sReach2[qb[it].type] * reachS;
-                            reach[it] = (base + g0 * g0 < r2max ? 1 : 0) | (base + g1 * g1 < r2max ? 2 : 0);
+                            reach[it] = (a0 + b0 < r2max ? 1 : 0) | (a1 + b0 < r2max ? 2 : 0) |
+                                        (a0 + b1 < r2max ? 4 : 0) | (a1 + b1 < r2max ? 8 : 0);
                         }
                         // forces stage imaged positions; gathers the stored ones (the image
                         // shift is applied per pair, as the reference's minImg)
@@ -444,31 +455,32 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         cd.c = kF ? src[it] : (c | ((int)src[it] << 5));
                         Cw[lane + it * kWave] = cd;
                     }
-                    // per half of the home cell, the staged candidates it can reach
-                    int nl0 = 0, nl1 = 0;
+                    // per quarter of the home cell, the staged candidates it can reach
+                    int nq[4] = {0, 0, 0, 0};
                     if (kCull) {
 #pragma unroll
                         for (int it = 0; it < kStageIters; it++) {
                             if (it >= iters) break;
-                            const bool r0 = reach[it] & 1, r1 = reach[it] & 2;
-                            const unsigned long long b0 = __ballot(r0), b1 = __ballot(r1);
-                            if (r0) sHalf[kCull ? wave : 0][0][nl0 + __popcll(b0 & lanes_lt)] = (uint8_t)(lane + it * kWave);
-                            if (r1) sHalf[kCull ? wave : 0][1][nl1 + __popcll(b1 & lanes_lt)] = (uint8_t)(lane + it * kWave);
-                            nl0 += __popcll(b0);
-                            nl1 += __popcll(b1);
+#pragma unroll
+                            for (int q = 0; q < 4; q++) {
+                                const bool r = (reach[it] >> q) & 1;
+                                const unsigned long long b = __ballot(r);
+                                if (r) mapW[q * kMaxC + nq[q] + __popcll(b & lanes_lt)] = (uint8_t)(lane + it * kWave);
+                                nq[q] += __popcll(b);
+                            }
                         }
                     }
                     wave_lds_sync();
                     if (MODE == M_FORCE1 && act) {
-                        // one phase over the home's half list: FP64 test, force law under
+                        // one phase over the home's quarter list: FP64 test, force law under
                         // the branch; two candidates per trip so both LDS reads are in
                         // flight before either is used
                         const float4 *C8 = sC8 + 2 * hb.type * nT;
-                        const int half = hfx >= 0.f ? 1 : 0;
-                        const uint8_t *lst = sHalf[kCull ? wave : 0][half];
-                        const int nl = half ? nl1 : nl0;
+                        const int quad = (hfx >= 0.f ? 1 : 0) + (hfy >= 0.f ? 2 : 0);
+                        const uint8_t *lst = mapW + quad * kMaxC;
+                        const int nl = quad == 0 ? nq[0] : (quad == 1 ? nq[1] : (quad == 2 ? nq[2] : nq[3]));
                         // the next trip's list entries are read one trip ahead (entries past
-                        // nl fall in the row's padding and are never used)
+                        // nl fall in the next list or the padding and are never used)
                         int j = sub;
                         const uint8_t *lp = lst + sub;
                         int i0 = lp[0], i1 = lp[tpp];
@@ -515,9 +527,9 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         // each pair once, from the side that owns it: only the home cell and
                         // its forward half shell are staged, and within the home cell the
                         // lower slot owns the pair
-                        const int half = hfx >= 0.f ? 1 : 0;
-                        const uint8_t *lst = sHalf[kCull ? wave : 0][half];
-                        const int nl = half ? nl1 : nl0;
+                        const int quad = (hfx >= 0.f ? 1 : 0) + (hfy >= 0.f ? 2 : 0);
+                        const uint8_t *lst = mapW + quad * kMaxC;
+                        const int nl = quad == 0 ? nq[0] : (quad == 1 ? nq[1] : (quad == 2 ? nq[2] : nq[3]));
                         const double *Ch = sCd + 6 * hb.type * nT;
                         for (int jj = sub; jj < nl; jj += tpp) {
                             const int i = lst[jj];
