@@ -47,10 +47,10 @@ void dfree(T *&p)
     p = nullptr;
 }
 
-enum KClass { K_FIRST, K_SORT, K_REORDER, K_PAIR, K_BONDED, K_GATHER, K_CHAIN_E, K_KIN, K_MC, K_NCLASS };
+enum KClass { K_FIRST, K_SORT, K_REORDER, K_PAIR, K_BONDED, K_GATHER, K_CHAIN_E, K_KIN, K_MC, K_KICK_FIRST, K_NCLASS };
 const char *kClassName[K_NCLASS] = {"integrate_first+key", "radix_sort", "reorder+cell_table", "pair_force",
                                     "bonded+langevin+kick", "pair_gather_fp64", "chain_energy_fp64",
-                                    "kinetic_fp64", "mc_move"};
+                                    "kinetic_fp64", "mc_move", "bonded+kick+next_first+key"};
 
 // k_first's step argument: set the device step counter, advance it by one, or leave it
 constexpr long long kStepAdvance = -1, kStepKeep = -2;
@@ -93,6 +93,7 @@ struct sm_ctx {
     std::vector<RepHost> rh;
     // device state
     Bead *beads = nullptr, *sbeads = nullptr;
+    Bead *beads_alt = nullptr;   // the fused kick + next first half writes positions here, then swaps
     float4 *vel = nullptr, *acc = nullptr, *fpair = nullptr, *fscratch = nullptr;
     uint32_t *k0 = nullptr, *v0 = nullptr, *k1 = nullptr, *v1 = nullptr;
     uint32_t *hist[2] = {nullptr, nullptr};   // sort scratch, double-buffered
@@ -149,7 +150,7 @@ struct sm_ctx {
         drop_graph();
         for (auto &r : recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
         for (auto e : pool) (void)hipEventDestroy(e);
-        dfree(beads); dfree(sbeads); dfree(vel); dfree(acc); dfree(fpair); dfree(fscratch);
+        dfree(beads); dfree(sbeads); dfree(beads_alt); dfree(vel); dfree(acc); dfree(fpair); dfree(fscratch);
         dfree(k0); dfree(v0); dfree(k1); dfree(v1); dfree(hist[0]); dfree(hist[1]); dfree(cstart); dfree(cend);
         dfree(reps); dfree(glob); dfree(fconstF); dfree(uconstD); dfree(chainPerBead); dfree(chainList);
         dfree(gptr); dfree(gent); dfree(gconstD);
@@ -383,6 +384,23 @@ struct sm_ctx {
     }
     bool mc_step(long long step) const { return step % 8 == 0 && step != 0 && any_mc; }
 
+    // Step s's bonded + Langevin forces and second half kick, with step s + 1's first half,
+    // keys and digit histograms, as one kernel (same arithmetic as bonded() then
+    // step_first(s + 1)); the new positions land in the other bead buffer.
+    void kick_first(long long s)
+    {
+        if (gexec[0] || gexec[1]) drop_graph();   // captured graphs hold the bead buffer
+        uint32_t *gh = hist[sort_parity];
+        if (hist_pending) launch_fill_u32(st, gh, sort_hist_words(), 0u);
+        hist_pending = true;
+        const int passes = sort_passes(sort_bits);
+        timed(K_KICK_FIRST, [&] {
+            launch_kick_first(st, P, beads, beads_alt, chainPerBead, cconstD, gen(), fpair, vel, acc, reps, glob, s,
+                              k0, v0, gh, passes);
+        });
+        std::swap(beads, beads_alt);
+    }
+
     void step_first(long long step_set) { first_and_keys(1, step_set); }
     void step_finish(bool measure, bool mc)
     {
@@ -437,8 +455,26 @@ struct sm_ctx {
                 graph_block(s);
                 s += 8;
             }
+            while (s < end) eager_step(s++);
+            return;
         }
-        while (s < end) eager_step(s++);
+        // eager: a step without a box move ends in the fused kernel that also does the next
+        // step's first half; a box-move step and the run's last step end unfused
+        bool first_done = false;
+        for (; s < end; s++) {
+            if (!first_done) step_first(s);
+            const bool mc = mc_step(s);
+            sort_and_reorder();
+            pair_force();
+            if (!mc && s + 1 < end) {
+                kick_first(s);
+                first_done = true;
+            } else {
+                bonded(KICK_STEP, nullptr);
+                if (mc) mc_move();
+                first_done = false;
+            }
+        }
     }
 };
 
@@ -680,6 +716,7 @@ sm_ctx *create(const sm_system *s, int nrep)
 
         c->beads = dalloc<Bead>(n);
         c->sbeads = dalloc<Bead>(n);
+        c->beads_alt = dalloc<Bead>(n);
         c->vel = dalloc<float4>(n);
         c->acc = dalloc<float4>(n);
         c->fpair = dalloc<float4>(n);

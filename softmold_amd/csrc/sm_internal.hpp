@@ -126,6 +126,12 @@ void launch_bonded_kick(hipStream_t st, const Params &P, const Bead *beads, cons
                         const double4 *cconst, const GenTables &gen, const float4 *fpair, float4 *vel, float4 *acc,
                         const DevRep *reps, const DevGlobal *glob, int mode, float4 *fchain_out);
 enum : int { KICK_STEP = 0, KICK_INIT = 1, KICK_NONE = 2 };
+// bonded + Langevin + second half kick of `step`, then Verlet::first, keys and digit
+// histograms of step + 1 (positions written to beads_out; the device step becomes step + 1)
+void launch_kick_first(hipStream_t st, const Params &P, const Bead *beads, Bead *beads_out, const int2 *chain,
+                       const double4 *cconst, const GenTables &gen, const float4 *fpair, float4 *vel, float4 *acc,
+                       const DevRep *reps, DevGlobal *glob, long long step, uint32_t *key, uint32_t *idx,
+                       uint32_t *ghist, int passes);
 void launch_second(hipStream_t st, int n, const Bead *beads, const float4 *acc, float4 *vel, float dt);
 
 // FP64 gathers over the 27-cell stencil.  GATHER_U / GATHER_DU write one FP64 value per

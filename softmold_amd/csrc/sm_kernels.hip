@@ -55,6 +55,40 @@ __device__ inline bool in_box(double p, double L) { return p >= 0.0 && p <= L; }
 // ---------------------------------------------------------------------------------------
 // Verlet::first (verlet.h:289-356) + cell key (cellOpt.h:530-556)
 // ---------------------------------------------------------------------------------------
+// Half kick v += a dt/2 (Verlet::first's and Verlet::second's velocity update); every
+// kernel that kicks uses this, so a kick rounds the same wherever it runs.
+__device__ inline void half_kick(float4 &v, const float4 &a, float halfdt)
+{
+    v.x = fmaf(a.x, halfdt, v.x);
+    v.y = fmaf(a.y, halfdt, v.y);
+    v.z = fmaf(a.z, halfdt, v.z);
+}
+
+// Verlet::first's drift x += v dt and the wrap into [0, L] (verlet.h:289-356)
+__device__ inline void drift_wrap(Bead &b, const float4 &v, double dt, double L0, double L1, double L2)
+{
+    b.x = fma((double)v.x, dt, b.x);
+    b.y = fma((double)v.y, dt, b.y);
+    b.z = fma((double)v.z, dt, b.z);
+    if (b.x > L0) b.x -= L0;
+    if (b.x < 0) b.x += L0;
+    if (b.y > L1) b.y -= L1;
+    if (b.y < 0) b.y += L1;
+    if (b.z > L2) b.z -= L2;
+    if (b.z < 0) b.z += L2;
+}
+
+// cell key of a bead (cellOpt.h:530-556), flagging a position outside the box
+__device__ inline uint32_t bead_key(const Bead &b, const DevRep &R, DevGlobal *__restrict__ glob)
+{
+    const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
+    if (!(in_box(b.x, R.L[0]) && in_box(b.y, R.L[1]) && in_box(b.z, R.L[2]))) atomicOr(&glob->flags, FLAG_BAD_POSITION);
+    const int cx = cell_axis(b.x, R.cs[0], nx);
+    const int cy = cell_axis(b.y, R.cs[1], ny);
+    const int cz = cell_axis(b.z, R.cs[2], nz);
+    return (uint32_t)(R.cellbase + (long long)(cx + cy * nx + cz * nx * ny));
+}
+
 // Verlet::first of bead i and its cell key
 __device__ inline uint32_t first_one(int i, Bead *__restrict__ beads, float4 *__restrict__ vel,
                                      const float4 *__restrict__ acc, const DevRep *__restrict__ reps,
@@ -62,31 +96,14 @@ __device__ inline uint32_t first_one(int i, Bead *__restrict__ beads, float4 *__
 {
     Bead b = beads[i];
     const DevRep &R = reps[b.rep];
-    const double L0 = R.L[0], L1 = R.L[1], L2 = R.L[2];
     if (integrate && b.type != 0) {
         float4 v = vel[i];
-        const float4 a = acc[i];
-        v.x += a.x * halfdt;
-        v.y += a.y * halfdt;
-        v.z += a.z * halfdt;
-        b.x += (double)v.x * dt;
-        b.y += (double)v.y * dt;
-        b.z += (double)v.z * dt;
-        if (b.x > L0) b.x -= L0;
-        if (b.x < 0) b.x += L0;
-        if (b.y > L1) b.y -= L1;
-        if (b.y < 0) b.y += L1;
-        if (b.z > L2) b.z -= L2;
-        if (b.z < 0) b.z += L2;
+        half_kick(v, acc[i], halfdt);
+        drift_wrap(b, v, dt, R.L[0], R.L[1], R.L[2]);
         vel[i] = v;
         beads[i] = b;
     }
-    const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
-    if (!(in_box(b.x, L0) && in_box(b.y, L1) && in_box(b.z, L2))) atomicOr(&glob->flags, FLAG_BAD_POSITION);
-    const int cx = cell_axis(b.x, R.cs[0], nx);
-    const int cy = cell_axis(b.y, R.cs[1], ny);
-    const int cz = cell_axis(b.z, R.cs[2], nz);
-    return (uint32_t)(R.cellbase + (long long)(cx + cy * nx + cz * nx * ny));
+    return bead_key(b, R, glob);
 }
 
 __global__ __launch_bounds__(kBlock) void k_first(int n, Bead *__restrict__ beads, float4 *__restrict__ vel,
@@ -180,19 +197,11 @@ __device__ inline double3 bend_f(double3 da, double3 db, double c0, double kt, i
     return make_double3(-B.x, -B.y, -B.z);
 }
 
-__global__ __launch_bounds__(kBlock) void k_bonded_kick(int n, const Bead *__restrict__ beads,
-                                                        const int2 *__restrict__ chain,
-                                                        const double4 *__restrict__ cconst, GenTables gen,
-                                                        const float4 *__restrict__ fpair, float4 *__restrict__ vel,
-                                                        float4 *__restrict__ acc, const DevRep *__restrict__ reps,
-                                                        const DevGlobal *__restrict__ glob, float halfdt, int mode,
-                                                        float4 *__restrict__ fchain_out)
+// CHAIN and generic bonded force on bead i (FP64)
+__device__ inline double3 bonded_force(int i, const Bead &p0, const Bead *__restrict__ beads,
+                                       const int2 *__restrict__ chain, const double4 *__restrict__ cconst,
+                                       const GenTables &gen, double L0, double L1, double L2)
 {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const Bead p0 = beads[i];
-    const DevRep &R = reps[p0.rep];
-    const double L0 = R.L[0], L1 = R.L[1], L2 = R.L[2];
     double3 fc = make_double3(0.0, 0.0, 0.0);
     const int2 ci = chain[i];
     if (ci.x >= 0) {
@@ -249,17 +258,16 @@ __global__ __launch_bounds__(kBlock) void k_bonded_kick(int n, const Bead *__res
             fc.x += f.x; fc.y += f.y; fc.z += f.z;
         }
     }
-    if (mode == KICK_NONE) {
-        fchain_out[i] = make_float4((float)fc.x, (float)fc.y, (float)fc.z, 0.f);
-        return;
-    }
-    const long long step = glob->step;
-    const uint4 o = philox((uint32_t)(i - R.first), (uint32_t)step, (uint32_t)((unsigned long long)step >> 32),
-                           mode == KICK_INIT ? TAG_LANGEVIN_INIT : TAG_LANGEVIN, (uint32_t)R.seed,
-                           (uint32_t)(R.seed >> 32));
+    return fc;
+}
+
+// total acceleration of bead i at `step`: pair + bonded + Langevin (-gamma v + sigma xi)
+__device__ inline float4 total_accel(int i, const DevRep &R, const double3 &fc, const float4 &fp, const float4 &v,
+                                     long long step, uint32_t tag)
+{
+    const uint4 o = philox((uint32_t)(i - R.first), (uint32_t)step, (uint32_t)((unsigned long long)step >> 32), tag,
+                           (uint32_t)R.seed, (uint32_t)(R.seed >> 32));
     const float gamma = (float)R.gamma, sigma = (float)R.sigma;
-    float4 v = vel[i];
-    const float4 fp = fpair[i];
     float4 a;
     a.x = (-gamma * v.x + sigma * psi32(o.x));
     a.y = (-gamma * v.y + sigma * psi32(o.y));
@@ -267,12 +275,79 @@ __global__ __launch_bounds__(kBlock) void k_bonded_kick(int n, const Bead *__res
     a.x += fp.x; a.y += fp.y; a.z += fp.z;
     a.x += (float)fc.x; a.y += (float)fc.y; a.z += (float)fc.z;
     a.w = 0.f;
+    return a;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bonded_kick(int n, const Bead *__restrict__ beads,
+                                                        const int2 *__restrict__ chain,
+                                                        const double4 *__restrict__ cconst, GenTables gen,
+                                                        const float4 *__restrict__ fpair, float4 *__restrict__ vel,
+                                                        float4 *__restrict__ acc, const DevRep *__restrict__ reps,
+                                                        const DevGlobal *__restrict__ glob, float halfdt, int mode,
+                                                        float4 *__restrict__ fchain_out)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const Bead p0 = beads[i];
+    const DevRep &R = reps[p0.rep];
+    const double3 fc = bonded_force(i, p0, beads, chain, cconst, gen, R.L[0], R.L[1], R.L[2]);
+    if (mode == KICK_NONE) {
+        fchain_out[i] = make_float4((float)fc.x, (float)fc.y, (float)fc.z, 0.f);
+        return;
+    }
+    float4 v = vel[i];
+    const float4 a = total_accel(i, R, fc, fpair[i], v, glob->step, mode == KICK_INIT ? TAG_LANGEVIN_INIT : TAG_LANGEVIN);
     acc[i] = a;
     if (mode == KICK_STEP && p0.type != 0) {
-        v.x += a.x * halfdt;
-        v.y += a.y * halfdt;
-        v.z += a.z * halfdt;
+        half_kick(v, a, halfdt);
         vel[i] = v;
+    }
+}
+
+// Step `step`'s bonded forces, Langevin force and second half kick, then Verlet::first and
+// the cell keys of step + 1, in one pass over the beads: the same arithmetic as
+// k_bonded_kick followed by k_first.  The drifted positions go to beads_out, since other
+// beads' bonded forces still read this step's positions; the step counter is taken as an
+// argument (no thread reads the device counter, which thread 0 advances to step + 1).
+__global__ __launch_bounds__(kBlock) void k_kick_first(int n, const Bead *__restrict__ beads,
+                                                       Bead *__restrict__ beads_out,
+                                                       const int2 *__restrict__ chain,
+                                                       const double4 *__restrict__ cconst, GenTables gen,
+                                                       const float4 *__restrict__ fpair, float4 *__restrict__ vel,
+                                                       float4 *__restrict__ acc, const DevRep *__restrict__ reps,
+                                                       DevGlobal *__restrict__ glob, float halfdt, double dt,
+                                                       long long step, uint32_t *__restrict__ key,
+                                                       uint32_t *__restrict__ idx, uint32_t *__restrict__ ghist,
+                                                       int passes)
+{
+    __shared__ uint32_t h[4][256];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i == 0) glob->step = step + 1;
+    for (int p = 0; p < passes; p++) h[p][threadIdx.x] = 0u;
+    __syncthreads();
+    uint32_t k = 0u;
+    if (i < n) {
+        Bead b = beads[i];
+        const DevRep &R = reps[b.rep];
+        const double3 fc = bonded_force(i, b, beads, chain, cconst, gen, R.L[0], R.L[1], R.L[2]);
+        float4 v = vel[i];
+        const float4 a = total_accel(i, R, fc, fpair[i], v, step, TAG_LANGEVIN);
+        acc[i] = a;
+        if (b.type != 0) {
+            half_kick(v, a, halfdt);   // Verlet::second of this step
+            half_kick(v, a, halfdt);   // Verlet::first of the next
+            drift_wrap(b, v, dt, R.L[0], R.L[1], R.L[2]);
+            vel[i] = v;
+        }
+        beads_out[i] = b;
+        k = bead_key(b, R, glob);
+    }
+    hist_add(h, k, i < n, passes);
+    __syncthreads();
+    hist_flush(h, ghist, passes);
+    if (i < n) {
+        key[i] = k;
+        idx[i] = (uint32_t)i;
     }
 }
 
@@ -284,10 +359,7 @@ __global__ __launch_bounds__(kBlock) void k_second(int n, const Bead *__restrict
     if (i >= n) return;
     if (beads[i].type == 0) return;
     float4 v = vel[i];
-    const float4 a = acc[i];
-    v.x += a.x * halfdt;
-    v.y += a.y * halfdt;
-    v.z += a.z * halfdt;
+    half_kick(v, acc[i], halfdt);
     vel[i] = v;
 }
 
@@ -669,6 +741,15 @@ void launch_bonded_kick(hipStream_t st, const Params &P, const Bead *beads, cons
 {
     hipLaunchKernelGGL(k_bonded_kick, dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, beads, chain, cconst, gen, fpair,
                        vel, acc, reps, glob, (float)(P.dt * 0.5), mode, fchain_out);
+}
+
+void launch_kick_first(hipStream_t st, const Params &P, const Bead *beads, Bead *beads_out, const int2 *chain,
+                       const double4 *cconst, const GenTables &gen, const float4 *fpair, float4 *vel, float4 *acc,
+                       const DevRep *reps, DevGlobal *glob, long long step, uint32_t *key, uint32_t *idx,
+                       uint32_t *ghist, int passes)
+{
+    hipLaunchKernelGGL(k_kick_first, dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, beads, beads_out, chain, cconst, gen,
+                       fpair, vel, acc, reps, glob, (float)(P.dt * 0.5), P.dt, step, key, idx, ghist, passes);
 }
 
 void launch_second(hipStream_t st, int n, const Bead *beads, const float4 *acc, float4 *vel, float halfdt)
