@@ -46,6 +46,11 @@ __global__ __launch_bounds__(kThreads) void k_onesweep(int n, const uint32_t *__
     const int t = threadIdx.x;
     const int wave = t / kWave;
     if (t == 0) s_tile = atomicAdd(tilectr, 1u);
+    // digit t's count over all keys, from every histogram copy: independent of the tile,
+    // so its loads are issued first and summed after the look-back
+    uint32_t gh[kHistCopies];
+#pragma unroll
+    for (int c = 0; c < kHistCopies; c++) gh[c] = ghist[(size_t)c * 4 * 256 + t];
     for (int r = 0; r < kRounds; r++)
         for (int w = 0; w < kWavesPerBlock; w++) wc[r][w][t] = 0u;
     __syncthreads();
@@ -53,13 +58,18 @@ __global__ __launch_bounds__(kThreads) void k_onesweep(int n, const uint32_t *__
     const int base = tile * kThreads * kRounds;
     const unsigned long long lt = lanes_below();
 
+    // every round's key and value loads in flight together (indices clamped in bounds;
+    // items past n are masked below)
     uint32_t kr[kRounds], vr[kRounds], dr[kRounds], rk[kRounds];
 #pragma unroll
     for (int r = 0; r < kRounds; r++) {
-        const int i = base + r * kThreads + t;
-        const bool valid = i < n;
-        kr[r] = valid ? kin[i] : 0u;
-        vr[r] = valid ? vin[i] : 0u;
+        const int i = min(base + r * kThreads + t, n - 1);
+        kr[r] = kin[i];
+        vr[r] = vin[i];
+    }
+#pragma unroll
+    for (int r = 0; r < kRounds; r++) {
+        const bool valid = base + r * kThreads + t < n;
         dr[r] = valid ? (kr[r] >> shift) & 255u : 256u;   // 256: no item
         const unsigned long long peers = match_digit(dr[r] & 255u, valid);
         rk[r] = (uint32_t)__popcll(peers & lt);
@@ -111,7 +121,8 @@ __global__ __launch_bounds__(kThreads) void k_onesweep(int n, const uint32_t *__
 
     // digit base: keys of smaller digits (all tiles) + this digit in earlier tiles
     uint32_t g = 0;
-    for (int c = 0; c < kHistCopies; c++) g += ghist[(size_t)c * 4 * 256 + t];
+#pragma unroll
+    for (int c = 0; c < kHistCopies; c++) g += gh[c];
     scan[t] = g;
     __syncthreads();
     for (int off = 1; off < 256; off <<= 1) {
