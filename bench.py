@@ -175,11 +175,13 @@ def bench_batched(steps: int, warmup: int, seq_steps: int) -> dict:
     log(f"batched: {len(systems)} replicas, {nb} beads, {1e3 * wall / steps:.3f} ms/step, "
         f"{batched:.4g} particle-steps/s, knobs {knobs}, MC acceptance {accepted.sum() / max(trials.sum(), 1):.3f}")
 
-    # the plain solver on each replica alone: same per-step work, summed walls
+    # the plain solver on each replica alone, in its default (eager) mode, where a step
+    # without a box move ends in the fused kick + next-first-half kernel: same per-step
+    # work, summed walls
     seq_wall, seq_work = 0.0, 0
     for d in systems:
         s = sm.Solver.from_system(d)
-        s.tune("graph", 1)
+        s.tune("graph", 0)
         s.init_forces()
         s.run(1, 16)
         s.sync()
