@@ -52,6 +52,37 @@ const char *kClassName[K_NCLASS] = {"integrate_first+key", "radix_sort", "reorde
                                     "bonded+langevin+kick", "pair_gather_fp64", "chain_energy_fp64",
                                     "kinetic_fp64", "mc_move", "bonded+kick+next_first+key"};
 
+// The pair kernels' constant table: each (home type, candidate type) row of the FP32 force
+// constants padded to 8 floats {r_m, C1, C2, r_c, C4, C5, 0, 0}, then the culling reach
+// r^2 of each candidate type for the force (from those FP32 constants) and for the FP64
+// energy gathers (from the FP64 potential constants), kMaxTypes floats each.  A candidate
+// type's reach is the largest over the home types of r_c when the pair has an outer branch
+// (C4 or C5 nonzero) and r_m otherwise, capped at the cutoff, plus 1e-4.
+std::vector<float> pair_table(const std::vector<double> &hf, const std::vector<double> &hu, int nT, double cutoff)
+{
+    const size_t rows = (size_t)nT * nT;
+    std::vector<float> tab(8 * rows + 2 * kMaxTypes, 0.f);
+    for (size_t r = 0; r < rows; r++)
+        for (int k = 0; k < 6; k++) tab[8 * r + k] = (float)hf[6 * r + k];
+    const double rcap = std::sqrt(cutoff * cutoff);
+    for (int t = 0; t < nT; t++) {
+        double rF = 0.0, rE = 0.0;
+        for (int h = 0; h < nT; h++) {
+            const size_t row = (size_t)h * nT + t;
+            const float *f = &tab[8 * row];
+            const double fm = f[0], fc = f[3], f4 = f[4], f5 = f[5];
+            rF = std::fmax(rF, (f4 != 0.0 || f5 != 0.0) ? fc : fm);
+            const double *u = &hu[6 * row];
+            rE = std::fmax(rE, (u[4] != 0.0 || u[5] != 0.0) ? u[3] : u[0]);
+        }
+        rF = std::fmin(rF, rcap) + 1e-4;
+        rE = std::fmin(rE, rcap) + 1e-4;
+        tab[8 * rows + t] = (float)(rF * rF);
+        tab[8 * rows + kMaxTypes + t] = (float)(rE * rE);
+    }
+    return tab;
+}
+
 // k_first's step argument: set the device step counter, advance it by one, or leave it
 constexpr long long kStepAdvance = -1, kStepKeep = -2;
 
@@ -105,7 +136,7 @@ struct sm_ctx {
     int sort_bits = 0;
     DevRep *reps = nullptr;
     DevGlobal *glob = nullptr;
-    float *fconstF = nullptr;
+    float *fconstF = nullptr;   // pair table: padded FP32 force rows, then the reach tables
     double *uconstD = nullptr;
     int2 *chainPerBead = nullptr, *chainList = nullptr;
     double4 *cconstD = nullptr;
@@ -339,7 +370,7 @@ struct sm_ctx {
     void measure_device()
     {
         timed(K_GATHER, [&] {
-            launch_pair_gather(st, P, chunk, GATHER_U, sbeads, skey(), sidx(), cstart, cend, reps, uconstD, ebead,
+            launch_pair_gather(st, P, chunk, GATHER_U, sbeads, skey(), sidx(), cstart, cend, reps, fconstF, uconstD, ebead,
                                nullptr, nullptr, nullptr, 0);
             launch_seg_sum(st, segB, ebead, meas_dev() + 0, 3);
         });
@@ -360,7 +391,7 @@ struct sm_ctx {
     void delta_device()
     {
         timed(K_GATHER, [&] {
-            launch_pair_gather(st, P, chunk, GATHER_DU, sbeads, skey(), sidx(), cstart, cend, reps, uconstD, ebead,
+            launch_pair_gather(st, P, chunk, GATHER_DU, sbeads, skey(), sidx(), cstart, cend, reps, fconstF, uconstD, ebead,
                                nullptr, nullptr, nullptr, 0);
             launch_seg_sum(st, segB, ebead, du_dev() + 0, 2);
         });
@@ -700,10 +731,10 @@ sm_ctx *create(const sm_system *s, int nrep)
             if (q.delta_lxy != 0) c->any_mc = true;
         }
         const int nT = s[0].n_types;
-        std::vector<float> fF(6 * nT * nT);
-        for (size_t k = 0; k < fF.size(); k++) fF[k] = (float)s[0].fconst[k];
-        c->hfconst.assign(s[0].fconst, s[0].fconst + fF.size());
-        c->huconst.assign(s[0].uconst, s[0].uconst + fF.size());
+        const size_t nconst = 6 * (size_t)nT * nT;
+        c->hfconst.assign(s[0].fconst, s[0].fconst + nconst);
+        c->huconst.assign(s[0].uconst, s[0].uconst + nconst);
+        const std::vector<float> fF = pair_table(c->hfconst, c->huconst, nT, s[0].cutoff);
         c->hcconst = ccD;
 
         // segment tiles: beads (molecule order == sorted order per replica) and chains
@@ -733,7 +764,7 @@ sm_ctx *create(const sm_system *s, int nrep)
         c->reps = dalloc<DevRep>(nrep);
         c->glob = dalloc<DevGlobal>(1);
         c->fconstF = dalloc<float>(fF.size());
-        c->uconstD = dalloc<double>(fF.size());
+        c->uconstD = dalloc<double>(nconst);
         c->chainPerBead = dalloc<int2>(n);
         c->chainList = dalloc<int2>(list.size());
         c->cconstD = dalloc<double4>(ccD.size());
@@ -765,7 +796,7 @@ sm_ctx *create(const sm_system *s, int nrep)
         HIPCHK(hipMemcpyAsync(c->reps, hr.data(), sizeof(DevRep) * nrep, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(c->glob, &g, sizeof g, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(c->fconstF, fF.data(), sizeof(float) * fF.size(), hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(c->uconstD, s[0].uconst, sizeof(double) * fF.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(c->uconstD, s[0].uconst, sizeof(double) * nconst, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(c->chainPerBead, perBead.data(), sizeof(int2) * n, hipMemcpyHostToDevice, st));
         if (!gentH.empty()) {
             c->gptr = dalloc<int>(gptrH.size());
@@ -1157,7 +1188,7 @@ int sm_pair_count(sm_ctx *c, int32_t *counts, int64_t *total)
         if (!c) invalid("null context");
         c->build(0, kStepKeep);
         launch_pair_gather(c->st, c->P, c->chunk, GATHER_COUNT, c->sbeads, c->skey(), c->sidx(), c->cstart, c->cend,
-                           c->reps, c->uconstD, nullptr, c->dcounts, nullptr, nullptr, 0);
+                           c->reps, c->fconstF, c->uconstD, nullptr, c->dcounts, nullptr, nullptr, 0);
         c->check_launch();
         std::vector<int32_t> h(c->n);
         HIPCHK(hipMemcpyAsync(h.data(), c->dcounts, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->st));
@@ -1182,7 +1213,7 @@ int sm_pair_list(sm_ctx *c, int32_t *pairs, int64_t cap, int64_t *n_out)
         c->build(0, kStepKeep);
         HIPCHK(hipMemsetAsync(c->dnp, 0, sizeof(unsigned long long), c->st));
         launch_pair_gather(c->st, c->P, c->chunk, GATHER_LIST, c->sbeads, c->skey(), c->sidx(), c->cstart, c->cend,
-                           c->reps, c->uconstD, nullptr, c->dcounts, c->dpairs, c->dnp, cap);
+                           c->reps, c->fconstF, c->uconstD, nullptr, c->dcounts, c->dpairs, c->dnp, cap);
         c->check_launch();
         unsigned long long np = 0;
         HIPCHK(hipMemcpyAsync(&np, c->dnp, sizeof np, hipMemcpyDeviceToHost, c->st));
@@ -1377,9 +1408,8 @@ void set_pair_rows(sm_ctx *c, int a, int b, double umin, double umax)
 void upload_constants(sm_ctx *c)
 {
     const size_t nt = c->hfconst.size();
-    std::vector<float> fF(nt);
-    for (size_t k = 0; k < nt; k++) fF[k] = (float)c->hfconst[k];
-    HIPCHK(hipMemcpyAsync(c->fconstF, fF.data(), sizeof(float) * nt, hipMemcpyHostToDevice, c->st));
+    const std::vector<float> fF = pair_table(c->hfconst, c->huconst, c->P.nT, c->P.cutoff);
+    HIPCHK(hipMemcpyAsync(c->fconstF, fF.data(), sizeof(float) * fF.size(), hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->uconstD, c->huconst.data(), sizeof(double) * nt, hipMemcpyHostToDevice, c->st));
     if (!c->hcconst.empty())
         HIPCHK(hipMemcpyAsync(c->cconstD, c->hcconst.data(), sizeof(double4) * c->hcconst.size(),

@@ -115,7 +115,7 @@ void launch_reorder(hipStream_t st, int n, const uint32_t *skey, const uint32_t 
 // the hits
 void launch_pair_force(hipStream_t st, const Params &P, int chunk, int variant, const Bead *sbeads,
                        const uint32_t *skey, const uint32_t *sidx, const int *cstart, const int *cend,
-                       const DevRep *reps, const float *fconst, float4 *fpair);
+                       const DevRep *reps, const float *tab, float4 *fpair);
 // gptr == NULL: CHAIN terms only; else also the generic records of each bead
 struct GenTables {
     const int *gptr;
@@ -142,8 +142,8 @@ enum : int { GATHER_U = 0, GATHER_DU = 1, GATHER_COUNT = 2, GATHER_LIST = 3 };
 int cells_blocks(int n, int chunk);       // workgroups of the cell traversals (sm_pair.hip)
 void launch_pair_gather(hipStream_t st, const Params &P, int chunk, int mode, const Bead *sbeads,
                         const uint32_t *skey, const uint32_t *sidx, const int *cstart, const int *cend,
-                        const DevRep *reps, const double *uconst, double *energy, int *counts, int *pairs,
-                        unsigned long long *npairs, long long cap);
+                        const DevRep *reps, const float *tab, const double *uconst, double *energy, int *counts,
+                        int *pairs, unsigned long long *npairs, long long cap);
 // One FP64 value per chain: its energy, or U(old) - U(scaled) when delta != 0.
 void launch_chain_energy(hipStream_t st, int nchains, const int2 *chains, const double4 *cconst, const Bead *beads,
                          const DevRep *reps, int delta, double *e);

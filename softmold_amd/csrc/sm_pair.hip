@@ -216,7 +216,7 @@ template <int MODE>
 __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(4))) void k_cells(
     int n, int nT, int chunk, int nblocks, double cut2, float cut2pre, const Bead *__restrict__ sb,
     const uint32_t *__restrict__ skey, const uint32_t *__restrict__ sidx, const int *__restrict__ cstart,
-    const int *__restrict__ cend, const DevRep *__restrict__ reps, const float *__restrict__ fconst,
+    const int *__restrict__ cend, const DevRep *__restrict__ reps, const float *__restrict__ tab,
     const double *__restrict__ uconst, Out out)
 {
     constexpr bool kF = MODE == M_FORCE || MODE == M_FORCE1;   // force (two-phase / one-phase)
@@ -257,39 +257,21 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     static_assert(kListBytes % 4 == 0, "the map is cleared in 32-bit words");
     __shared__ uint32_t sLists[kWaves][kListBytes / 4];
 
+    // constant rows and culling reach from the pair table that the host builds: the FP32
+    // force rows padded to 8 floats, then the reach r^2 per candidate type for the force
+    // and for the FP64 energy gathers.  The reach of a candidate type is r_c where some
+    // pair with it has an outer branch (Umin != 0 in laradjiRevalee.h:97-110), else r_m:
+    // beyond r_m such pairs have zero force and potential, so the candidates culled by it
+    // contribute exactly zero.
+    __shared__ float sReach2[kMaxTypes];
     if (kF) {
-        float *c8 = reinterpret_cast<float *>(sC8);
-        for (int t = threadIdx.x; t < 8 * nT * nT; t += blockDim.x) c8[t] = (t & 7) < 6 ? fconst[6 * (t >> 3) + (t & 7)] : 0.f;
+        const float4 *t8 = reinterpret_cast<const float4 *>(tab);
+        for (int t = threadIdx.x; t < 2 * nT * nT; t += blockDim.x) sC8[t] = t8[t];
     } else {
         for (int t = threadIdx.x; t < 6 * nT * nT; t += blockDim.x) sCd[t] = uconst[t];
     }
+    if (kCull && (int)threadIdx.x < nT) sReach2[threadIdx.x] = tab[8 * nT * nT + (kE ? kMaxTypes : 0) + threadIdx.x];
     __syncthreads();
-    // Culling radius per candidate type: a type pair whose outer coefficients are zero
-    // (Umin = 0 in laradjiRevalee.h:97-110, every pair but tail-tail in the lipid models)
-    // has zero force and potential beyond r_m, so a candidate of type t matters only within
-    // the largest such reach over the home types: r_c where some pair with t has an outer
-    // branch, else r_m.  Candidates culled by it contribute exactly zero.
-    __shared__ float sReach2[kMaxTypes];
-    if (kCull) {
-        if ((int)threadIdx.x < nT) {
-            const int t = threadIdx.x;
-            double r = 0.0;
-            for (int h = 0; h < nT; h++) {
-                const int row = 6 * (h * nT + t);
-                double rm, rc, c4, c5;
-                if (kF) {
-                    const float *c8 = reinterpret_cast<const float *>(sC8) + 8 * (h * nT + t);
-                    rm = c8[0]; rc = c8[3]; c4 = c8[4]; c5 = c8[5];
-                } else {
-                    rm = sCd[row]; rc = sCd[row + 3]; c4 = sCd[row + 4]; c5 = sCd[row + 5];
-                }
-                r = fmax(r, (c4 != 0.0 || c5 != 0.0) ? rc : rm);
-            }
-            r = fmin(r, sqrt(cut2)) + 1e-4;
-            sReach2[t] = (float)(r * r);
-        }
-        __syncthreads();
-    }
 
     const int lane = threadIdx.x & (kWave - 1);
     const unsigned long long lanes_lt = lane ? (~0ull >> (64 - lane)) : 0ull;
@@ -668,17 +650,17 @@ void launch_pair_force(hipStream_t st, const Params &P, int chunk, int variant, 
 
 void launch_pair_gather(hipStream_t st, const Params &P, int chunk, int mode, const Bead *sbeads,
                         const uint32_t *skey, const uint32_t *sidx, const int *cstart, const int *cend,
-                        const DevRep *reps, const double *uconst, double *energy, int *counts, int *pairs,
-                        unsigned long long *npairs, long long cap)
+                        const DevRep *reps, const float *tab, const double *uconst, double *energy, int *counts,
+                        int *pairs, unsigned long long *npairs, long long cap)
 {
     Out o{nullptr, energy, counts, pairs, npairs, cap};
     switch (mode) {
-    case GATHER_U: launch_cells<M_U>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, nullptr, uconst, o); break;
-    case GATHER_DU: launch_cells<M_DU>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, nullptr, uconst, o); break;
+    case GATHER_U: launch_cells<M_U>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, tab, uconst, o); break;
+    case GATHER_DU: launch_cells<M_DU>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, tab, uconst, o); break;
     case GATHER_COUNT:
-        launch_cells<M_COUNT>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, nullptr, uconst, o);
+        launch_cells<M_COUNT>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, tab, uconst, o);
         break;
-    default: launch_cells<M_LIST>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, nullptr, uconst, o); break;
+    default: launch_cells<M_LIST>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, tab, uconst, o); break;
     }
 }
 
