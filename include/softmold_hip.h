@@ -153,13 +153,13 @@ int sm_potential_delta(sm_ctx *ctx, const double *scale, double *du_pair, double
 /* Set a launch-shape knob of this build; no counterpart in the reference (its launches are
  * fixed, include/cuda/other.h:6-11).  Knobs: "pair_chunk" = sorted slots per wave64 in
  * the pair-force kernel (1..4096); "pair_variant" = 0 one-phase force kernel with the
- * home cell's candidates culled per half cell (default), 1 two-phase (FP32 screen into
+ * home cell's candidates culled per quarter cell (default), 1 two-phase (FP32 screen into
  * per-lane hit lists, then the FP64 test on the hits);
  * "graph" = 1 to replay sm_run's aligned 8-step blocks as one captured hipGraph. */
 int sm_tune(sm_ctx *ctx, const char *knob, int32_t value);
 int sm_tune_get(sm_ctx *ctx, const char *knob, int32_t *value);
 /* Autotuner (no reference counterpart): times `steps` steps of this context's own work
- * for each pair_chunk in {4,8,16,32,64} with the one-phase kernel and the graph on, then
+ * for each pair_chunk in {2,4,8,16,32,64} with the one-phase kernel and the graph on, then
  * the best without the graph, from one saved state; keeps the fastest setting, restores
  * the state, and returns the best ms per step.  These settings leave every result
  * bit-identical, so tuning never changes a trajectory. */

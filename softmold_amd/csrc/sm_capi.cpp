@@ -812,9 +812,9 @@ sm_ctx *create(const sm_system *s, int nrep)
             HIPCHK(hipMemcpyAsync(c->cconstD, ccD.data(), sizeof(double4) * ccD.size(), hipMemcpyHostToDevice, st));
         }
         // sorted slots per wave in the pair kernel: short chunks keep small systems filling
-        // the chip, long ones cut the per-cell overhead of large ones (measured on the
-        // 12k, 15k, 240k and 3M-bead systems; every chunk gives the same forces)
-        c->chunk = n >= 1000000 ? 16 : (n >= 50000 ? 8 : 4);
+        // the chip, longer ones cut the per-cell overhead of large ones (measured on the
+        // 15k, 240k and 3M-bead systems; every chunk gives the same forces)
+        c->chunk = n >= 50000 ? 8 : 2;
         c->ensure_capacity(0);
         HIPCHK(hipStreamSynchronize(st));
     } catch (...) {
