@@ -275,7 +275,8 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
 
     const int lane = threadIdx.x & (kWave - 1);
     const unsigned long long lanes_lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-    const int wave = threadIdx.x / kWave;
+    // wave index, wave-uniform (the compiler then keeps the chunk's cell setup scalar)
+    const int wave = kWaves == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
     const long long a0 = ((long long)xcd_block(blockIdx.x, nblocks) * kWaves + wave) * chunk;
     const int a = (int)min(a0, (long long)n);
     const int aend = min(a + chunk, n);
