@@ -305,3 +305,26 @@ def test_device_ball_matches_oracle():
     st = s.bond_stats()
     assert np.isnan(st[0]) and list(st[1:]) == [0.0, 0.0, 0.0]     # BALL is not a bond for lBond_
     s.close()
+
+
+@pytest.mark.gpu
+def test_run_equals_step_phases_with_records():
+    """sm_run ends each step without a box move in the fused kick + next-first-half kernel;
+    stepping phase by phase never fuses.  With BOND, BEND and BALL records and box moves
+    every 8 steps, both give the same state bit for bit."""
+    if not gpu_available():
+        pytest.skip("no GPU")
+    d = _with_ball(_crosslinked(sm.Mpd.bilayer(13, 400, 3.45).system()))
+    d.update(delta_lxy=0.05, tension=0.5)
+    a, b = sm.Solver.from_system(d), sm.Solver.from_system(d)
+    a.init_forces()
+    b.init_forces()
+    a.run(0, 19)
+    for i in range(19):
+        b.step_first(i)
+        b.step_finish(i)
+    for x, y in zip(a.download(), b.download()):
+        assert np.array_equal(x, y)
+    assert a.mc_stats() == b.mc_stats()
+    a.close()
+    b.close()
