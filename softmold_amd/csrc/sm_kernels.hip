@@ -171,20 +171,21 @@ __device__ inline double3 min_image(const Bead &p, const Bead &q, double L0, dou
     return make_double3(dx, dy, dz);
 }
 
-// harmonic force on the first particle of the bond (a1 += d * mag)
-__device__ inline double3 bond_f(double3 d, double l0, double kb)
+__device__ inline double len3(const double3 &d) { return sqrt(d.x * d.x + d.y * d.y + d.z * d.z); }
+
+// harmonic force on the first particle of the bond (a1 += d * mag), with |d| given
+__device__ inline double3 bond_fl(double3 d, double dr, double l0, double kb)
 {
-    const double dr = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
     const double mag = -(dr - l0) * kb / dr;
     return make_double3(d.x * mag, d.y * mag, d.z * mag);
 }
 
-// bend: returns the force on the particle with role 0, 1 or 2 of (p1, p2, p3)
-__device__ inline double3 bend_f(double3 da, double3 db, double c0, double kt, int role)
+__device__ inline double3 bond_f(double3 d, double l0, double kb) { return bond_fl(d, len3(d), l0, kb); }
+
+// bend: the force on the particle with role 0, 1 or 2 of (p1, p2, p3), given the two
+// differences and their inverse lengths
+__device__ inline double3 bend_fl(double3 da, double ia, double3 db, double ib, double c0, double kt, int role)
 {
-    const double dra = sqrt(da.x * da.x + da.y * da.y + da.z * da.z);
-    const double drb = sqrt(db.x * db.x + db.y * db.y + db.z * db.z);
-    const double ia = 1.0 / dra, ib = 1.0 / drb;
     da.x *= ia; da.y *= ia; da.z *= ia;
     db.x *= ib; db.y *= ib; db.z *= ib;
     const double cost = da.x * db.x + da.y * db.y + da.z * db.z;
@@ -195,6 +196,11 @@ __device__ inline double3 bend_f(double3 da, double3 db, double c0, double kt, i
     if (role == 0) return A;
     if (role == 1) return make_double3(B.x - A.x, B.y - A.y, B.z - A.z);
     return make_double3(-B.x, -B.y, -B.z);
+}
+
+__device__ inline double3 bend_f(double3 da, double3 db, double c0, double kt, int role)
+{
+    return bend_fl(da, 1.0 / len3(da), db, 1.0 / len3(db), c0, kt, role);
 }
 
 // CHAIN and generic bonded force on bead i (FP64)
@@ -213,24 +219,34 @@ __device__ inline double3 bonded_force(int i, const Bead &p0, const Bead *__rest
         if (j >= 2) pm2 = beads[i - 2];
         if (j <= len - 2) pp1 = beads[i + 1];
         if (j <= len - 3) pp2 = beads[i + 2];
+        // the chain differences around this bead, each with its length and inverse length
+        // computed once (the bonds and bends below share them)
+        const double3 z3 = make_double3(0.0, 0.0, 0.0);
+        const double3 dm2 = j >= 2 ? min_image(pm2, pm1, L0, L1, L2) : z3;        // p(i-2) - p(i-1)
+        const double3 dm1 = j >= 1 ? min_image(pm1, p0, L0, L1, L2) : z3;         // p(i-1) - p(i)
+        const double3 d0 = j <= len - 2 ? min_image(p0, pp1, L0, L1, L2) : z3;    // p(i) - p(i+1)
+        const double3 d1 = j <= len - 3 ? min_image(pp1, pp2, L0, L1, L2) : z3;   // p(i+1) - p(i+2)
+        const double rm2 = j >= 2 ? len3(dm2) : 1.0, rm1 = j >= 1 ? len3(dm1) : 1.0;
+        const double r0 = j <= len - 2 ? len3(d0) : 1.0, r1 = j <= len - 3 ? len3(d1) : 1.0;
         if (j >= 1) {  // bond (i-1, i): this bead is a2
-            const double3 f = bond_f(min_image(pm1, p0, L0, L1, L2), K.x, K.y);
+            const double3 f = bond_fl(dm1, rm1, K.x, K.y);
             fc.x -= f.x; fc.y -= f.y; fc.z -= f.z;
         }
         if (j <= len - 2) {  // bond (i, i+1): this bead is a1
-            const double3 f = bond_f(min_image(p0, pp1, L0, L1, L2), K.x, K.y);
+            const double3 f = bond_fl(d0, r0, K.x, K.y);
             fc.x += f.x; fc.y += f.y; fc.z += f.z;
         }
+        const double im2 = 1.0 / rm2, im1 = 1.0 / rm1, i0 = 1.0 / r0, i1 = 1.0 / r1;
         if (j <= len - 3) {  // bend (i, i+1, i+2), role 0
-            const double3 f = bend_f(min_image(p0, pp1, L0, L1, L2), min_image(pp1, pp2, L0, L1, L2), K.z, K.w, 0);
+            const double3 f = bend_fl(d0, i0, d1, i1, K.z, K.w, 0);
             fc.x += f.x; fc.y += f.y; fc.z += f.z;
         }
         if (j >= 1 && j <= len - 2) {  // bend (i-1, i, i+1), role 1
-            const double3 f = bend_f(min_image(pm1, p0, L0, L1, L2), min_image(p0, pp1, L0, L1, L2), K.z, K.w, 1);
+            const double3 f = bend_fl(dm1, im1, d0, i0, K.z, K.w, 1);
             fc.x += f.x; fc.y += f.y; fc.z += f.z;
         }
         if (j >= 2) {  // bend (i-2, i-1, i), role 2
-            const double3 f = bend_f(min_image(pm2, pm1, L0, L1, L2), min_image(pm1, p0, L0, L1, L2), K.z, K.w, 2);
+            const double3 f = bend_fl(dm2, im2, dm1, im1, K.z, K.w, 2);
             fc.x += f.x; fc.y += f.y; fc.z += f.z;
         }
     }
