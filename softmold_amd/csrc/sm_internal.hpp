@@ -110,7 +110,7 @@ void launch_first(hipStream_t st, const Params &P, Bead *beads, float4 *vel, con
 // also zeroes clear[0 .. clear_words) (the next sort's scratch)
 void launch_reorder(hipStream_t st, int n, const uint32_t *skey, const uint32_t *sidx, const Bead *beads,
                     Bead *sbeads, int *cstart, int *cend, uint32_t *clear, size_t clear_words);
-// variant 0: one phase (FP64 test, force law under the branch) over per-half-cell culled
+// variant 0: one phase (FP64 test, force law under the branch) over per-quarter-cell culled
 // candidate lists; 1: FP32 screen into a per-lane hit list, then FP64 test + force law on
 // the hits
 void launch_pair_force(hipStream_t st, const Params &P, int chunk, int variant, const Bead *sbeads,
