@@ -15,7 +15,8 @@
 //      per-cell LDS tables), so all global loads of a pass are in flight together; a
 //      pass holds kMaxC candidates and runs only the 64-slot rounds it fills;
 //   3. tpp = 64/H lanes serve each of the H home particles, each lane walking every
-//      tpp-th candidate; partial results meet in LDS and are combined in lane order.
+//      tpp-th candidate; a home's partial results are combined by a fixed-shape tree of
+//      shuffles (at step s, lane sub adds lane sub + s when sub is a multiple of 2s).
 // Force, one phase (M_FORCE1, the default): candidates are staged as FP64 imaged
 // positions, and each staged candidate is entered into the list of each quarter of the home
 // cell (x below / above the cell centre times y below / above) whose box it comes within
@@ -232,16 +233,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     static_assert(kMaxC % kWave == 0, "staging rounds cover whole waves");
     __shared__ float4 sC8[kF ? 2 * kMaxTypes * kMaxTypes : 1];   // rows padded to 8 floats
     __shared__ double sCd[kF ? 1 : 6 * kMaxTypes * kMaxTypes];
-    // staged candidates; after a chunk's passes the same memory holds the per-lane partial
-    // results that are combined per home particle (LDS per workgroup bounds the resident
-    // waves, so the reduction takes no space of its own)
-    union WaveBuf {
-        Cand cand[kMaxC];
-        float f[3][kWave];
-        double d[kWave];
-        int i[kWave];
-    };
-    __shared__ WaveBuf sBuf[kWaves];
+    __shared__ Cand sCand[kWaves][kMaxC];   // staged candidates of the current pass
     __shared__ float4 sRel[k2P ? kWaves : 1][k2P ? kMaxC : 1];
     __shared__ uint8_t sList[k2P ? kWaves : 1][k2P ? kList : 1][kWave];   // per-lane hit lists, lane-minor
     // per stencil cell: image shift, and slot in the sorted order minus slot in the pass
@@ -281,8 +273,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     const int a = (int)min(a0, (long long)n);
     const int aend = min(a + chunk, n);
     constexpr uint32_t fwd = forward_mask();
-    WaveBuf &Bw = sBuf[wave];
-    Cand *Cw = Bw.cand;
+    Cand *Cw = sCand[wave];
     float4 *Rw = sRel[k2P ? wave : 0];
 
     for (int base = a; base < aend; base += kWave) {
@@ -572,39 +563,30 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                     wave_lds_sync();
                 }
                 if (kF) {
-                    Bw.f[0][lane] = fx;
-                    Bw.f[1][lane] = fy;
-                    Bw.f[2][lane] = fz;
-                    wave_lds_sync();
-                    if (act && sub == 0) {
-                        float sx = 0.f, sy = 0.f, sz = 0.f;
-                        for (int t = 0; t < tpp; t++) {
-                            sx += Bw.f[0][lane + t];
-                            sy += Bw.f[1][lane + t];
-                            sz += Bw.f[2][lane + t];
+                    // a home's tpp lanes combined by a fixed-shape tree: at step s, lane sub
+                    // adds lane sub + s when sub is a multiple of 2s
+                    for (int st = 1; st < tpp; st <<= 1) {
+                        const float ox = __shfl(fx, lane + st), oy = __shfl(fy, lane + st), oz = __shfl(fz, lane + st);
+                        if ((sub & (2 * st - 1)) == 0 && sub + st < tpp) {
+                            fx += ox;
+                            fy += oy;
+                            fz += oz;
                         }
-                        out.force[me] = make_float4(sx, sy, sz, 0.f);
                     }
-                    wave_lds_sync();
+                    if (act && sub == 0) out.force[me] = make_float4(fx, fy, fz, 0.f);
                 } else if (MODE == M_COUNT) {
-                    Bw.i[lane] = cnt;
-                    wave_lds_sync();
-                    if (act && sub == 0) {
-                        int c = 0;
-                        for (int t = 0; t < tpp; t++) c += Bw.i[lane + t];
-                        out.counts[me] = c;
+                    for (int st = 1; st < tpp; st <<= 1) {
+                        const int o = __shfl(cnt, lane + st);
+                        if ((sub & (2 * st - 1)) == 0 && sub + st < tpp) cnt += o;
                     }
-                    wave_lds_sync();
+                    if (act && sub == 0) out.counts[me] = cnt;
                 } else if (MODE == M_U || MODE == M_DU) {
-                    // the home particle's share, its lanes combined in lane order
-                    Bw.d[lane] = accu;
-                    wave_lds_sync();
-                    if (act && sub == 0) {
-                        double e = 0.0;
-                        for (int t = 0; t < tpp; t++) e += Bw.d[lane + t];
-                        out.energy[s0 + h0 + hi] = e;
+                    // the home particle's share, its lanes combined by the same fixed tree
+                    for (int st = 1; st < tpp; st <<= 1) {
+                        const double o = __shfl(accu, lane + st);
+                        if ((sub & (2 * st - 1)) == 0 && sub + st < tpp) accu += o;
                     }
-                    wave_lds_sync();
+                    if (act && sub == 0) out.energy[s0 + h0 + hi] = accu;
                 }
             }
         }
