@@ -86,12 +86,23 @@ __device__ inline void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs; map them so each XCD gets one
-// contiguous range of the sorted particles (shared stencil cells stay in one L2).
+// Workgroups are dispatched round-robin over the 8 XCDs.  G == 0: each XCD gets one
+// contiguous range of the sorted particles (shared stencil cells stay in one L2); G > 0:
+// the XCDs take turns at groups of G consecutive chunks (the last partial round of groups
+// in order), which balances systems whose work is uneven along the sort order -- the
+// half-shell energy gathers over a flat bilayer ran 19 % faster, while the force over a
+// vesicle ran 5 % slower.
+template <int G>
 __device__ inline int xcd_block(int b, int nblocks)
 {
-    const int x = b & 7, q = nblocks >> 3, r = nblocks & 7;
-    return x * q + min(x, r) + (b >> 3);
+    if (G == 0) {
+        const int x = b & 7, q = nblocks >> 3, r = nblocks & 7;
+        return x * q + min(x, r) + (b >> 3);
+    }
+    const int full = (nblocks / (8 * G)) * (8 * G);
+    if (b >= full) return b;
+    const int x = b & 7, k = b >> 3;
+    return ((k / (G > 0 ? G : 1)) * 8 + x) * G + (k % (G > 0 ? G : 1));
 }
 
 // r^2 rounded as the reference computes it, (x*x + y*y) + z*z with every product and sum
@@ -269,7 +280,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     const unsigned long long lanes_lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     // wave index, wave-uniform (the compiler then keeps the chunk's cell setup scalar)
     const int wave = kWaves == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
-    const long long a0 = ((long long)xcd_block(blockIdx.x, nblocks) * kWaves + wave) * chunk;
+    const long long a0 = ((long long)xcd_block<kE ? 64 : 0>(blockIdx.x, nblocks) * kWaves + wave) * chunk;
     const int a = (int)min(a0, (long long)n);
     const int aend = min(a + chunk, n);
     constexpr uint32_t fwd = forward_mask();
