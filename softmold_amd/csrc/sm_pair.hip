@@ -429,11 +429,13 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                             reach[it] = (a0 + b0 < r2max ? 1 : 0) | (a1 + b0 < r2max ? 2 : 0) |
                                         (a0 + b1 < r2max ? 4 : 0) | (a1 + b1 < r2max ? 8 : 0);
                         }
-                        // forces stage imaged positions; gathers the stored ones (the image
-                        // shift is applied per pair, as the reference's minImg)
-                        cd.x = kF ? ix : qb[it].x;
-                        cd.y = kF ? iy : qb[it].y;
-                        cd.z = kF ? iz : qb[it].z;
+                        // forces and energy gathers stage imaged positions (q + shift, the
+                        // one rounding the reference's minImg makes of the same operands);
+                        // the count and list gathers the stored ones, since they take each
+                        // pair's difference from either side
+                        cd.x = (kF || kE) ? ix : qb[it].x;
+                        cd.y = (kF || kE) ? iy : qb[it].y;
+                        cd.z = (kF || kE) ? iz : qb[it].z;
                         cd.t = qb[it].type;
                         // force: the candidate's sorted slot (the self test reads it with the
                         // type, in the same 16-byte load); gathers: stencil cell | slot << 5
@@ -522,10 +524,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                             const Cand q = Cw[i];
                             const int c = q.c & 31;
                             if (j == self || (c == 13 && j < self)) continue;
-                            const Shift sh = ShW[c];
-                            const double px = q.x + sh.x;
-                            const double py = q.y + sh.y;
-                            const double pz = q.z + sh.z;
+                            const double px = q.x, py = q.y, pz = q.z;   // staged imaged
                             const double *C = Ch + 6 * q.t;
                             const double r2 = sq3(hb.x - px, hb.y - py, hb.z - pz);
                             if (MODE == M_U) {
