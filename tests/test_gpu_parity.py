@@ -15,7 +15,7 @@ from conftest import gpu_available
 pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not gpu_available(), reason="needs the MI355X")]
 
-FORCE_TOL = 1e-5      # per-particle, floored
+FORCE_TOL = 1e-5      # per-particle, floored (the step-0 force bar for FP32 pair arithmetic)
 FORCE_TOL_L2 = 1e-6   # global L2
 
 
