@@ -142,6 +142,13 @@ int sm_cell_keys(sm_ctx *ctx, int32_t *keys);
 /* Neighbour count per particle with the r^2 < rc^2 test in FP64 (the FORCE_COUNT build
  * of Force, MD.h:838-845); total = number of pairs. */
 int sm_pair_count(sm_ctx *ctx, int32_t *counts, int64_t *total);
+/* The pair force kernel's own accepted set, counted per particle: its traversal, staging,
+ * per-quarter-cell culling and FP64 test r^2 < rc^2, with a pair counted when its force
+ * term is not identically zero -- r^2 < r_m^2, or a row with a nonzero outer coefficient
+ * (Force, MD.h:811-848; Umin != 0 in laradjiRevalee.h:97-110).  A parity hook for the
+ * culling: the same definition over all pairs of the FORCE_COUNT traversal (MD.h:838-845)
+ * must give the same counts.  Needs the one-phase force kernel (pair_variant 0). */
+int sm_force_pair_count(sm_ctx *ctx, int32_t *counts, int64_t *total);
 /* Pairs (i<j) within the cutoff, FP64 test, in no particular order. */
 int sm_pair_list(sm_ctx *ctx, int32_t *pairs, int64_t cap, int64_t *n_out);
 /* dU = U(old) - U(scaled) over the current pair set, FP64 (CellOpt::computeDPotential

@@ -260,7 +260,10 @@ static int32_t neighbour(const cells_t *c, const double box[3], int32_t key, int
     return cells_find(c, k);
 }
 
-/* Force<double> (include/MD.h:795-848); count != 0 gives the FORCE_COUNT variant. */
+/* Force<double> (include/MD.h:795-848); count 1 gives the FORCE_COUNT variant
+ * (MD.h:838-845): every pair within the cutoff.  count 2 counts only the pairs whose force
+ * term is not identically zero: r^2 < r_m^2 (the inner branch), or an outer branch with a
+ * nonzero coefficient (row entries 4, 5; Umin != 0 in laradjiRevalee.h:97-110). */
 static inline void pair_force(double cut2, int nT, const double *p1, int t1, const double *p2,
                               int t2, double *a1, double *a2, const double *fC, int count)
 {
@@ -269,6 +272,10 @@ static inline void pair_force(double cut2, int nT, const double *p1, int t1, con
     double dz = p1[2] - p2[2];
     double dr = dx * dx + dy * dy + dz * dz;
     if (dr < cut2) {
+        if (count == 2) {
+            const int ci = 6 * (t1 * nT + t2);
+            if (!(fC[ci + 4] != 0.0 || fC[ci + 5] != 0.0 || dr < fC[ci] * fC[ci])) return;
+        }
         if (count) {
             a1[0] += 1; a1[1] += 1; a1[2] += 1;
             a2[0] += 1; a2[1] += 1; a2[2] += 1;
@@ -312,7 +319,8 @@ static inline double pair_potential(double cut2, int nT, const double *p1, int t
  * (cellOpt.h:712-926, 928-1041, 1043-1180): per occupied cell, the intra-cell triangle
  * with local copies, then the 13 forward neighbours with the periodic image added to the
  * neighbour's position.  mode 0: forces, 1: counts, 2: potential, 3: dPotential,
- * 4: pair list.  Forces and counts run cells in parallel with per-cell locks like the
+ * 4: pair list, 5: counts of the pairs with a nonzero force term.  Forces and counts run
+ * cells in parallel with per-cell locks like the
  * reference; with one thread the arithmetic order is the reference's single-thread order. */
 typedef struct {
     int mode;
@@ -328,6 +336,7 @@ static double traverse(const orc_sys *s, trav_t *t)
     cells_build(s, &c);
     const double cut2 = s->cutoff * s->cutoff;
     const int nT = s->ntypes;
+    const int fcount = t->mode == 1 ? 1 : (t->mode == 5 ? 2 : 0);
     double total = 0.0;
     int maxn = 1;
     for (int i = 0; i < c.ncell; i++) {
@@ -336,12 +345,12 @@ static double traverse(const orc_sys *s, trav_t *t)
     }
 #ifdef _OPENMP
     omp_lock_t *locks = NULL;
-    int use_locks = (t->mode <= 1);
+    int use_locks = (t->mode <= 1 || t->mode == 5);
     if (use_locks) {
         locks = (omp_lock_t *)malloc(sizeof(omp_lock_t) * (size_t)(c.ncell > 0 ? c.ncell : 1));
         for (int i = 0; i < c.ncell; i++) omp_init_lock(&locks[i]);
     }
-    int parallel = (t->mode <= 3);
+    int parallel = (t->mode <= 3 || t->mode == 5);
 #pragma omp parallel if (parallel) reduction(+ : total)
 #endif
     {
@@ -362,9 +371,9 @@ static double traverse(const orc_sys *s, trav_t *t)
                 ti[ni] = m;
                 ta[3 * ni] = ta[3 * ni + 1] = ta[3 * ni + 2] = 0.0;
                 for (int j = 0; j < ni; j++) {
-                    if (t->mode <= 1) {
+                    if (t->mode <= 1 || t->mode == 5) {
                         pair_force(cut2, nT, tp + 3 * ni, tt[ni], tp + 3 * j, tt[j], ta + 3 * ni,
-                                   ta + 3 * j, s->fconst, t->mode);
+                                   ta + 3 * j, s->fconst, fcount);
                     } else if (t->mode == 2) {
                         total += pair_potential(cut2, nT, tp + 3 * ni, tt[ni], tp + 3 * j, tt[j], s->uconst);
                     } else if (t->mode == 3) {
@@ -402,10 +411,10 @@ static double traverse(const orc_sys *s, trav_t *t)
                     cp[1] = s->pos[3 * (size_t)n + 1] + img[1];
                     cp[2] = s->pos[3 * (size_t)n + 2] + img[2];
                     int tn = s->type[n];
-                    if (t->mode <= 1) {
+                    if (t->mode <= 1 || t->mode == 5) {
                         double ca[3] = {0.0, 0.0, 0.0};
                         for (int m = 0; m < ni; m++)
-                            pair_force(cut2, nT, tp + 3 * m, tt[m], cp, tn, ta + 3 * m, ca, s->fconst, t->mode);
+                            pair_force(cut2, nT, tp + 3 * m, tt[m], cp, tn, ta + 3 * m, ca, s->fconst, fcount);
                         t->acc[3 * (size_t)n] += ca[0];
                         t->acc[3 * (size_t)n + 1] += ca[1];
                         t->acc[3 * (size_t)n + 2] += ca[2];
@@ -439,7 +448,7 @@ static double traverse(const orc_sys *s, trav_t *t)
                 if (use_locks) omp_unset_lock(&locks[nb]);
 #endif
             }
-            if (t->mode <= 1) {
+            if (t->mode <= 1 || t->mode == 5) {
 #ifdef _OPENMP
                 if (use_locks) omp_set_lock(&locks[ci]);
 #endif
@@ -475,6 +484,15 @@ void orc_pair_count(const orc_sys *s, int32_t *counts)
 {
     double *acc = (double *)calloc(3 * (size_t)s->n + 1, sizeof(double));
     trav_t t = {1, acc, NULL, NULL, 0, 0};
+    traverse(s, &t);
+    for (int i = 0; i < s->n; i++) counts[i] = (int32_t)acc[3 * (size_t)i];
+    free(acc);
+}
+
+void orc_pair_count_active(const orc_sys *s, int32_t *counts)
+{
+    double *acc = (double *)calloc(3 * (size_t)s->n + 1, sizeof(double));
+    trav_t t = {5, acc, NULL, NULL, 0, 0};
     traverse(s, &t);
     for (int i = 0; i < s->n; i++) counts[i] = (int32_t)acc[3 * (size_t)i];
     free(acc);

@@ -69,6 +69,9 @@ void orc_cell_keys(const orc_sys *s, int32_t *keys);
  * reference's single-thread order. */
 void orc_pair_force(const orc_sys *s, double *acc);          /* acc += F_pair */
 void orc_pair_count(const orc_sys *s, int32_t *counts);      /* FORCE_COUNT analogue */
+/* pairs within the cutoff whose force term is not identically zero: r^2 < r_m^2, or a row
+ * with a nonzero outer coefficient (F row entries 4 or 5) */
+void orc_pair_count_active(const orc_sys *s, int32_t *counts);
 int64_t orc_pair_list(const orc_sys *s, int32_t *pairs, int64_t cap); /* (i,j), i<j */
 double orc_pair_potential(const orc_sys *s);
 double orc_pair_dpotential(const orc_sys *s, const double scale[3]);

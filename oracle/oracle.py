@@ -74,6 +74,7 @@ def _declare(L):
         "orc_cell_keys": (None, [P(OrcSys), P(i32)]),
         "orc_pair_force": (None, [P(OrcSys), P(d)]),
         "orc_pair_count": (None, [P(OrcSys), P(i32)]),
+        "orc_pair_count_active": (None, [P(OrcSys), P(i32)]),
         "orc_pair_list": (i64, [P(OrcSys), P(i32), i64]),
         "orc_pair_potential": (d, [P(OrcSys)]),
         "orc_pair_dpotential": (d, [P(OrcSys), P(d)]),
@@ -280,6 +281,13 @@ class System:
     def pair_count(self):
         c = np.zeros(self.n, np.int32)
         lib().orc_pair_count(self.ref(), _ip(c))
+        return c
+
+    def pair_count_active(self):
+        """Per particle: partners within the cutoff whose force term is not identically zero
+        (r^2 < r_m^2, or a pair row with a nonzero outer coefficient)."""
+        c = np.zeros(self.n, np.int32)
+        lib().orc_pair_count_active(self.ref(), _ip(c))
         return c
 
     def pair_list(self):

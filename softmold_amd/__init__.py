@@ -55,7 +55,7 @@ EXPORTS = [
     "sm_last_error", "sm_version", "sm_create", "sm_create_batch", "sm_destroy", "sm_replicas",
     "sm_sync", "sm_download_state", "sm_upload_state", "sm_set_temperature", "sm_init_forces",
     "sm_second_half", "sm_step_first", "sm_step_finish", "sm_run", "sm_mc_stats", "sm_mc_hist",
-    "sm_measure", "sm_vesicle_radius", "sm_bond_stats", "sm_compute_forces", "sm_cell_keys", "sm_pair_count", "sm_pair_list",
+    "sm_measure", "sm_vesicle_radius", "sm_bond_stats", "sm_compute_forces", "sm_cell_keys", "sm_pair_count", "sm_force_pair_count", "sm_pair_list",
     "sm_potential_delta", "sm_tune", "sm_tune_get", "sm_autotune", "sm_set_params", "sm_get_params",
     "sm_adjoint", "sm_profile_enable",
     "sm_profile_read", "sm_mpd_read", "sm_mpd_write", "sm_mpd_free", "sm_mpd_system",
@@ -105,6 +105,7 @@ def _declare(L):
         "sm_compute_forces": (C.c_int, [vp, P(d), P(d)]),
         "sm_cell_keys": (C.c_int, [vp, P(i32)]),
         "sm_pair_count": (C.c_int, [vp, P(i32), P(i64)]),
+        "sm_force_pair_count": (C.c_int, [vp, P(i32), P(i64)]),
         "sm_pair_list": (C.c_int, [vp, P(i32), i64, P(i64)]),
         "sm_potential_delta": (C.c_int, [vp, P(d), P(d), P(d)]),
         "sm_tune": (C.c_int, [vp, C.c_char_p, i32]),
@@ -428,6 +429,14 @@ class Solver:
         c = np.zeros(self.n, np.int32)
         t = C.c_int64()
         _check(lib().sm_pair_count(self.h, _ip(c), C.byref(t)))
+        return c, t.value
+
+    def force_pair_count(self):
+        """Per particle: the pair force kernel's own accepted partners with a nonzero force
+        term (its culling and FP64 test), and the number of such pairs."""
+        c = np.zeros(self.n, np.int32)
+        t = C.c_int64()
+        _check(lib().sm_force_pair_count(self.h, _ip(c), C.byref(t)))
         return c, t.value
 
     def pair_list(self):

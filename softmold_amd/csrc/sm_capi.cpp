@@ -138,6 +138,7 @@ struct sm_ctx {
     DevGlobal *glob = nullptr;
     float *fconstF = nullptr;   // pair table: padded FP32 force rows, then the reach tables
     double *uconstD = nullptr;
+    double *fconstD = nullptr;   // FP64 force rows (the force kernel's pair-count hook)
     int2 *chainPerBead = nullptr, *chainList = nullptr;
     double4 *cconstD = nullptr;
     // generic BOND / BEND records, per bead (CSR); gptr == nullptr when there are none
@@ -183,7 +184,7 @@ struct sm_ctx {
         for (auto e : pool) (void)hipEventDestroy(e);
         dfree(beads); dfree(sbeads); dfree(beads_alt); dfree(vel); dfree(acc); dfree(fpair); dfree(fscratch);
         dfree(k0); dfree(v0); dfree(k1); dfree(v1); dfree(hist[0]); dfree(hist[1]); dfree(cstart); dfree(cend);
-        dfree(reps); dfree(glob); dfree(fconstF); dfree(uconstD); dfree(chainPerBead); dfree(chainList);
+        dfree(reps); dfree(glob); dfree(fconstF); dfree(uconstD); dfree(fconstD); dfree(chainPerBead); dfree(chainList);
         dfree(gptr); dfree(gent); dfree(gconstD);
         dfree(cconstD); dfree(ebead); dfree(echain); dfree(dres); dfree(dcounts); dfree(dpairs); dfree(dnp);
         dfree(adjF); dfree(adjDF); dfree(adjDU);
@@ -765,6 +766,7 @@ sm_ctx *create(const sm_system *s, int nrep)
         c->glob = dalloc<DevGlobal>(1);
         c->fconstF = dalloc<float>(fF.size());
         c->uconstD = dalloc<double>(nconst);
+        c->fconstD = dalloc<double>(nconst);
         c->chainPerBead = dalloc<int2>(n);
         c->chainList = dalloc<int2>(list.size());
         c->cconstD = dalloc<double4>(ccD.size());
@@ -797,6 +799,7 @@ sm_ctx *create(const sm_system *s, int nrep)
         HIPCHK(hipMemcpyAsync(c->glob, &g, sizeof g, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(c->fconstF, fF.data(), sizeof(float) * fF.size(), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(c->uconstD, s[0].uconst, sizeof(double) * nconst, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(c->fconstD, s[0].fconst, sizeof(double) * nconst, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(c->chainPerBead, perBead.data(), sizeof(int2) * n, hipMemcpyHostToDevice, st));
         if (!gentH.empty()) {
             c->gptr = dalloc<int>(gptrH.size());
@@ -1201,6 +1204,26 @@ int sm_pair_count(sm_ctx *c, int32_t *counts, int64_t *total)
     });
 }
 
+int sm_force_pair_count(sm_ctx *c, int32_t *counts, int64_t *total)
+{
+    return guarded([&] {
+        if (!c) invalid("null context");
+        if (c->pair_variant != 0) unsupported("the force pair count follows the one-phase force kernel (pair_variant 0)");
+        c->build(0, kStepKeep);
+        launch_pair_gather(c->st, c->P, c->chunk, GATHER_FCOUNT, c->sbeads, c->skey(), c->sidx(), c->cstart, c->cend,
+                           c->reps, c->fconstF, c->fconstD, nullptr, c->dcounts, nullptr, nullptr, 0);
+        c->check_launch();
+        std::vector<int32_t> h(c->n);
+        HIPCHK(hipMemcpyAsync(h.data(), c->dcounts, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        long long t = 0;
+        for (int i = 0; i < c->n; i++) t += h[i];
+        if (counts) std::memcpy(counts, h.data(), sizeof(int32_t) * c->n);
+        if (total) *total = t / 2;
+        c->check_flags();
+    });
+}
+
 int sm_pair_list(sm_ctx *c, int32_t *pairs, int64_t cap, int64_t *n_out)
 {
     return guarded([&] {
@@ -1411,6 +1434,7 @@ void upload_constants(sm_ctx *c)
     const std::vector<float> fF = pair_table(c->hfconst, c->huconst, c->P.nT, c->P.cutoff);
     HIPCHK(hipMemcpyAsync(c->fconstF, fF.data(), sizeof(float) * fF.size(), hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->uconstD, c->huconst.data(), sizeof(double) * nt, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->fconstD, c->hfconst.data(), sizeof(double) * nt, hipMemcpyHostToDevice, c->st));
     if (!c->hcconst.empty())
         HIPCHK(hipMemcpyAsync(c->cconstD, c->hcconst.data(), sizeof(double4) * c->hcconst.size(),
                               hipMemcpyHostToDevice, c->st));

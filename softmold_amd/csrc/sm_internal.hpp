@@ -137,8 +137,10 @@ void launch_second(hipStream_t st, int n, const Bead *beads, const float4 *acc, 
 // FP64 gathers over the 27-cell stencil.  GATHER_U / GATHER_DU write one FP64 value per
 // sorted slot into energy[] (the slot's share of U, or of U(old) - U(scaled) with each
 // replica's trial scale); GATHER_COUNT writes neighbour counts in molecule order;
-// GATHER_LIST appends pairs.
-enum : int { GATHER_U = 0, GATHER_DU = 1, GATHER_COUNT = 2, GATHER_LIST = 3 };
+// GATHER_LIST appends pairs; GATHER_FCOUNT runs the one-phase force kernel's traversal
+// (its staging, culling and FP64 test) and writes, in molecule order, the count of its
+// accepted pairs whose force term is not identically zero (uconst = the FP64 force rows).
+enum : int { GATHER_U = 0, GATHER_DU = 1, GATHER_COUNT = 2, GATHER_LIST = 3, GATHER_FCOUNT = 4 };
 int cells_blocks(int n, int chunk);       // workgroups of the cell traversals (sm_pair.hip)
 void launch_pair_gather(hipStream_t st, const Params &P, int chunk, int mode, const Bead *sbeads,
                         const uint32_t *skey, const uint32_t *sidx, const int *cstart, const int *cend,

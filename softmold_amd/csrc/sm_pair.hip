@@ -42,7 +42,7 @@ namespace {
 constexpr int kWaves = 1;   // waves per workgroup: one, so a finished wave frees its slot at once
 constexpr int kList = 32;      // candidates a lane screens before evaluating its hits
 
-enum : int { M_FORCE = 0, M_U = 1, M_DU = 2, M_COUNT = 3, M_LIST = 4, M_FORCE1 = 5 };
+enum : int { M_FORCE = 0, M_U = 1, M_DU = 2, M_COUNT = 3, M_LIST = 4, M_FORCE1 = 5, M_FCOUNT = 6 };
 
 struct __align__(16) Cand {
     double x, y, z;
@@ -215,6 +215,19 @@ __device__ inline void force_term(const Bead &hb, const Cand &q, bool other, dou
     }
 }
 
+// The force kernel's candidate as a count (M_FCOUNT): the same FP64 test as force_term,
+// then the pair counted when its force term is not identically zero -- r^2 < r_m^2, or a row
+// with a nonzero outer coefficient (Umin != 0, laradjiRevalee.h:97-110).  Cd holds the home
+// type's FP64 force rows {r_m, C1, C2, r_c, C4, C5}.
+__device__ inline void count_term(const Bead &hb, const Cand &q, bool other, double cut2, const double *Cd, int &cnt)
+{
+    const double r2 = sq3(hb.x - q.x, hb.y - q.y, hb.z - q.z);
+    if (r2 < cut2 && other && (unsigned)q.t < (unsigned)kMaxTypes) {
+        const double *C = Cd + 6 * q.t;
+        if (C[4] != 0.0 || C[5] != 0.0 || r2 < C[0] * C[0]) cnt++;
+    }
+}
+
 struct Out {
     float4 *force;            // M_FORCE, original order
     double *energy;           // M_U, M_DU: one FP64 value per sorted slot
@@ -235,7 +248,10 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     constexpr bool k2P = MODE == M_FORCE;
     constexpr bool kF1 = MODE == M_FORCE1;
     constexpr bool kE = MODE == M_U || MODE == M_DU;   // FP64 energy gathers
-    constexpr bool kCull = kF1 || kE;                  // candidates culled per quarter of the home cell
+    // the one-phase force traversal counting its accepted pairs with a nonzero term
+    // instead of evaluating them: same staging, culling, lists and FP64 test
+    constexpr bool kFC = MODE == M_FCOUNT;
+    constexpr bool kCull = kF1 || kE || kFC;           // candidates culled per quarter of the home cell
     // staged candidates per pass: the force kernel measured fastest at 192; the two-phase
     // force and the FP64 energy gathers at 128 (their larger constant tables and registers
     // leave less LDS per resident wave)
@@ -433,13 +449,16 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         // one rounding the reference's minImg makes of the same operands);
                         // the count and list gathers the stored ones, since they take each
                         // pair's difference from either side
-                        cd.x = (kF || kE) ? ix : qb[it].x;
-                        cd.y = (kF || kE) ? iy : qb[it].y;
-                        cd.z = (kF || kE) ? iz : qb[it].z;
-                        cd.t = qb[it].type;
+                        cd.x = (kF || kE || kFC) ? ix : qb[it].x;
+                        cd.y = (kF || kE || kFC) ? iy : qb[it].y;
+                        cd.z = (kF || kE || kFC) ? iz : qb[it].z;
                         // force: the candidate's sorted slot (the self test reads it with the
-                        // type, in the same 16-byte load); gathers: stencil cell | slot << 5
-                        cd.c = kF ? src[it] : (c | ((int)src[it] << 5));
+                        // type, in the same 16-byte load); energy gathers: its stencil cell;
+                        // count and list gathers: the slot, with the stencil cell above the
+                        // type's bits (types are below kMaxTypes <= 256)
+                        constexpr bool kSlotCell = MODE == M_COUNT || MODE == M_LIST;
+                        cd.t = kSlotCell ? (qb[it].type | (c << 8)) : qb[it].type;
+                        cd.c = (kF || kFC || kSlotCell) ? src[it] : c;
                         Cw[lane + it * kWave] = cd;
                     }
                     // per quarter of the home cell, the staged candidates it can reach
@@ -458,11 +477,12 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         }
                     }
                     wave_lds_sync();
-                    if (MODE == M_FORCE1 && act) {
+                    if ((MODE == M_FORCE1 || kFC) && act) {
                         // one phase over the home's quarter list: FP64 test, force law under
                         // the branch; two candidates per trip so both LDS reads are in
                         // flight before either is used
                         const float4 *C8 = sC8 + 2 * hb.type * nT;
+                        const double *Cd = sCd + 6 * hb.type * nT;
                         const int quad = (hfx >= 0.f ? 1 : 0) + (hfy >= 0.f ? 2 : 0);
                         const uint8_t *lst = mapW + quad * kMaxC;
                         const int nl = quad == 0 ? nq[0] : (quad == 1 ? nq[1] : (quad == 2 ? nq[2] : nq[3]));
@@ -476,14 +496,22 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                             const int n0 = lp[0], n1 = lp[tpp];
                             const Cand q0 = load_cand(Cw + i0);
                             const Cand q1 = load_cand(Cw + i1);
-                            force_term(hb, q0, q0.c != hslot, cut2, C8, fx, fy, fz);
-                            force_term(hb, q1, q1.c != hslot, cut2, C8, fx, fy, fz);
+                            if (kFC) {
+                                count_term(hb, q0, q0.c != hslot, cut2, Cd, cnt);
+                                count_term(hb, q1, q1.c != hslot, cut2, Cd, cnt);
+                            } else {
+                                force_term(hb, q0, q0.c != hslot, cut2, C8, fx, fy, fz);
+                                force_term(hb, q1, q1.c != hslot, cut2, C8, fx, fy, fz);
+                            }
                             i0 = n0;
                             i1 = n1;
                         }
                         if (j < nl) {
                             const Cand q0 = load_cand(Cw + i0);
-                            force_term(hb, q0, q0.c != hslot, cut2, C8, fx, fy, fz);
+                            if (kFC)
+                                count_term(hb, q0, q0.c != hslot, cut2, Cd, cnt);
+                            else
+                                force_term(hb, q0, q0.c != hslot, cut2, C8, fx, fy, fz);
                         }
                     } else if (k2P) {
                         const float4 *C8 = sC8 + 2 * hb.type * nT;
@@ -522,7 +550,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                             const int i = lst[jj];
                             const int j = cb + i;
                             const Cand q = Cw[i];
-                            const int c = q.c & 31;
+                            const int c = q.c;
                             if (j == self || (c == 13 && j < self)) continue;
                             const double px = q.x, py = q.y, pz = q.z;   // staged imaged
                             const double *C = Ch + 6 * q.t;
@@ -542,7 +570,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         for (int j = cb + sub; j < ce; j += tpp) {
                             if (j == self) continue;
                             const Cand q = Cw[j - cb];
-                            const int c = q.c & 31;
+                            const int c = q.t >> 8;
                             const Shift sh = ShW[c];
                             const double sx = sh.x, sy = sh.y, sz = sh.z;
                             const bool own = (c == 13) || ((fwd >> c) & 1u);
@@ -558,7 +586,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                             if (r2 < cut2) {
                                 cnt++;
                                 if (MODE == M_LIST) {
-                                    const uint32_t other = sidx[q.c >> 5];
+                                    const uint32_t other = sidx[q.c];
                                     if (me < other) {
                                         const unsigned long long slot = atomicAdd(out.npairs, 1ull);
                                         if ((long long)slot < out.cap) {
@@ -584,7 +612,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         }
                     }
                     if (act && sub == 0) out.force[me] = make_float4(fx, fy, fz, 0.f);
-                } else if (MODE == M_COUNT) {
+                } else if (MODE == M_COUNT || kFC) {
                     for (int st = 1; st < tpp; st <<= 1) {
                         const int o = __shfl(cnt, lane + st);
                         if ((sub & (2 * st - 1)) == 0 && sub + st < tpp) cnt += o;
@@ -652,6 +680,9 @@ void launch_pair_gather(hipStream_t st, const Params &P, int chunk, int mode, co
     case GATHER_DU: launch_cells<M_DU>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, tab, uconst, o); break;
     case GATHER_COUNT:
         launch_cells<M_COUNT>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, tab, uconst, o);
+        break;
+    case GATHER_FCOUNT:
+        launch_cells<M_FCOUNT>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, tab, uconst, o);
         break;
     default: launch_cells<M_LIST>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, tab, uconst, o); break;
     }

@@ -88,6 +88,27 @@ def test_neighbour_counts_bit_exact(name):
     assert total == rc.sum() // 2
 
 
+@pytest.mark.parametrize("name", ["L500", "B400", "L5000", "B4000", "L80000", "B1M", "dense"])
+def test_force_kernel_pair_set_bit_exact(name):
+    """The pair force kernel's own accepted set -- its staging, per-quarter-cell culling and
+    FP64 test, counting the pairs whose force term is not identically zero -- equals the
+    oracle's count of such pairs over its full FORCE_COUNT traversal, per particle, at the
+    default and at a long chunk.  A culling that dropped a pair with a nonzero term would
+    show here as a count one short."""
+    d = dense_system() if name == "dense" else system(name)
+    print(f"case force_pair_set {name} N={len(d['type'])}", flush=True)
+    s = sm.Solver.from_system(d)
+    ref = oracle_of(d).pair_count_active()
+    for chunk in (None, 64):
+        if chunk:
+            s.tune("pair_chunk", chunk)
+        c, total = s.force_pair_count()
+        np.testing.assert_array_equal(c, ref)
+        assert total == ref.sum() // 2
+    # the culling-free count is larger wherever head pairs sit between r_m and rc
+    assert s.pair_count()[1] > total
+
+
 @pytest.mark.parametrize("name", ["L500", "B400", "L5000", "B4000", "L80000", "B1M"])
 def test_step0_pair_forces_fp32_vs_fp64(name):
     d = system(name)
@@ -151,9 +172,9 @@ def test_potential_delta_fp64(name):
         assert dc == pytest.approx(o.chain_dpotential(scale), rel=1e-8, abs=1e-12)
 
 
-def test_dense_cells_and_boundary_particles():
+def dense_system():
     """Cells with more than 64 particles and stencils beyond one LDS pass; particles at
-    exactly 0 and L."""
+    exactly 0 and L; head and tail types mixed."""
     rng = np.random.default_rng(9)
     n = 3000
     box = np.array([12.0, 12.0, 12.0])
@@ -164,6 +185,11 @@ def test_dense_cells_and_boundary_particles():
     d = dict(system("L500"))
     d.update(pos=pos, type=types, vel=np.zeros((n, 3)), box=box,
              chain=np.zeros((0, 3), np.int32), chain_const=np.zeros((0, 4)))
+    return d
+
+
+def test_dense_cells_and_boundary_particles():
+    d = dense_system()
     s = sm.Solver.from_system(d)
     o = oracle_of(d)
     np.testing.assert_array_equal(s.cell_keys(), o.cell_keys())

@@ -116,6 +116,28 @@ def test_pair_counts_match_reference_force_count(golden_step0, golden_traj, name
 
 
 @pytest.mark.parametrize("name", ["L500", "B400t"])
+def test_active_pair_count_equals_brute_force(golden_traj, name):
+    """The oracle's count of pairs with a nonzero force term (the set the device force
+    kernel's own count is compared with) against an O(N^2) evaluation of its definition:
+    r^2 < rc^2 and (r^2 < r_m^2 or an outer coefficient of the row nonzero)."""
+    s, _ = _system(golden_traj, name)
+    nT = int(round(np.sqrt(len(s.fconst) / 6)))
+    F = s.fconst.reshape(nT * nT, 6)
+    d = s.pos[:, None, :] - s.pos[None, :, :]
+    d -= s.box * np.round(d / s.box)
+    r2 = (d ** 2).sum(-1)
+    row = s.type[:, None] * nT + s.type[None, :]
+    outer = (F[row, 4] != 0.0) | (F[row, 5] != 0.0)
+    act = (r2 < CUT * CUT) & (outer | (r2 < F[row, 0] ** 2))
+    np.fill_diagonal(act, False)
+    got = s.pair_count_active()
+    np.testing.assert_array_equal(got, act.sum(1))
+    # a strict subset of the FORCE_COUNT pairs: head pairs between r_m and rc drop out
+    allc = s.pair_count()
+    assert np.all(got <= allc) and got.sum() < allc.sum()
+
+
+@pytest.mark.parametrize("name", ["L500", "B400t"])
 def test_pair_list_equals_brute_force(golden_traj, name):
     s, _ = _system(golden_traj, name)
     pl = s.pair_list()
