@@ -128,6 +128,14 @@ int sm_measure(sm_ctx *ctx, double *out);
  * centre-of-mass loop images heads towards the replica's first bead; the radius loop adds
  * its image term).  radius holds one value per replica (NaN for a replica without heads). */
 int sm_vesicle_radius(sm_ctx *ctx, int32_t head_type, double *radius);
+/* sm_vesicle_radius without any device allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under stream capture, tight measuring
+ * loops of a sweep driver, harnesses that forbid hipMalloc).  Its partial sums live in a
+ * fixed per-GPU device array of 8 doubles per replica, set up at context creation, so it
+ * accepts contexts of at most 1024 replicas (SM_ERR_UNSUPPORTED beyond).  Results equal
+ * sm_vesicle_radius's bit for bit.  All calls on one GPU share that array: calls that
+ * overlap on one GPU (other host threads, other contexts' streams) race. */
+int sm_vesicle_radius_noalloc(sm_ctx *ctx, int32_t head_type, double *radius);
 /* Statistics of the generic records at the current state, 4 per replica, as MD writes them
  * to lBond_name.dat and bend_name.dat (dataExtraction.h:861-930,1009-1027): mean bond
  * length (0/0 = NaN without BOND records), then the bends' mean cos theta and mean arm

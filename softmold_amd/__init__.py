@@ -55,7 +55,7 @@ EXPORTS = [
     "sm_last_error", "sm_version", "sm_create", "sm_create_batch", "sm_destroy", "sm_replicas",
     "sm_sync", "sm_download_state", "sm_upload_state", "sm_set_temperature", "sm_init_forces",
     "sm_second_half", "sm_step_first", "sm_step_finish", "sm_run", "sm_mc_stats", "sm_mc_hist",
-    "sm_measure", "sm_vesicle_radius", "sm_bond_stats", "sm_compute_forces", "sm_cell_keys", "sm_pair_count", "sm_force_pair_count", "sm_pair_list",
+    "sm_measure", "sm_vesicle_radius", "sm_vesicle_radius_noalloc", "sm_bond_stats", "sm_compute_forces", "sm_cell_keys", "sm_pair_count", "sm_force_pair_count", "sm_pair_list",
     "sm_potential_delta", "sm_tune", "sm_tune_get", "sm_autotune", "sm_set_params", "sm_get_params",
     "sm_adjoint", "sm_profile_enable",
     "sm_profile_read", "sm_mpd_read", "sm_mpd_write", "sm_mpd_free", "sm_mpd_system",
@@ -101,6 +101,7 @@ def _declare(L):
         "sm_mc_hist": (C.c_int, [vp, i32, i32, P(d), P(d), P(i32)]),
         "sm_measure": (C.c_int, [vp, P(d)]),
         "sm_vesicle_radius": (C.c_int, [vp, i32, P(d)]),
+        "sm_vesicle_radius_noalloc": (C.c_int, [vp, i32, P(d)]),
         "sm_bond_stats": (C.c_int, [vp, P(d)]),
         "sm_compute_forces": (C.c_int, [vp, P(d), P(d)]),
         "sm_cell_keys": (C.c_int, [vp, P(i32)]),
@@ -400,11 +401,13 @@ class Solver:
         _check(lib().sm_measure(self.h, _dp(out)))
         return self._one(out)
 
-    def vesicle_radius(self, head_type: int = 2):
+    def vesicle_radius(self, head_type: int = 2, noalloc: bool = False):
         """Mean distance of the head beads from their centre of mass, per replica, computed
-        on the device (analysis/vesicleRadius.cpp:71-175)."""
+        on the device (analysis/vesicleRadius.cpp:71-175).  noalloc: through the entry point
+        that allocates no device memory (at most 1024 replicas)."""
         out = np.zeros(self.nrep)
-        _check(lib().sm_vesicle_radius(self.h, int(head_type), _dp(out)))
+        fn = lib().sm_vesicle_radius_noalloc if noalloc else lib().sm_vesicle_radius
+        _check(fn(self.h, int(head_type), _dp(out)))
         return self._one(out)
 
     def bond_stats(self):

@@ -28,7 +28,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result",
           f"-I{os.path.join(os.path.dirname(HERE), 'include')}"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-LIB_SOURCES = ["sm_kernels.hip", "sm_pair.hip", "sm_sort.hip", "sm_adjoint.hip", "sm_capi.cpp", "sm_mpd_api.cpp",
+LIB_SOURCES = ["sm_kernels.hip", "sm_pair.hip", "sm_sort.hip", "sm_adjoint.hip", "sm_noalloc.hip", "sm_capi.cpp", "sm_mpd_api.cpp",
                "mpd_io.cpp"]
 
 

@@ -116,6 +116,7 @@ SegHost make_tiles(const std::vector<int> &begin, const std::vector<int> &len)
 
 struct sm_ctx {
     hipStream_t st = nullptr;
+    int device = 0;   // the device current at creation, whose stream this is
     Params P{};
     int n = 0;
     int nrep = 0;
@@ -606,6 +607,10 @@ sm_ctx *create(const sm_system *s, int nrep)
         c->P.cutoff = s[0].cutoff;
         c->P.dt = s[0].dt;
         HIPCHK(hipStreamCreate(&c->st));
+        // the device's fixed scratch for the entry points that allocate nothing, looked up
+        // once per device while it is current
+        HIPCHK(hipGetDevice(&c->device));
+        HIPCHK(noalloc_scratch_init(c->device));
 
         // chains: per-bead role table and per-chain list, replica after replica
         std::vector<int2> perBead(n, make_int2(-1, 0));
@@ -1152,6 +1157,34 @@ int sm_vesicle_radius(sm_ctx *c, int32_t head_type, double *radius)
             throw;
         }
         dfree(w);
+    });
+}
+
+int sm_vesicle_radius_noalloc(sm_ctx *c, int32_t head_type, double *radius)
+{
+    return guarded([&] {
+        if (!c || !radius) invalid("null argument");
+        const int R = c->nrep;
+        if (R > kNoAllocMaxReplicas)
+            unsupported("sm_vesicle_radius_noalloc: " + std::to_string(R) + " replicas exceed its fixed scratch of " +
+                        std::to_string(kNoAllocMaxReplicas) + "; use sm_vesicle_radius");
+        double *w = noalloc_scratch(c->device);
+        if (!w) throw ApiError(SM_ERR_STATE, "fixed scratch of this device was not set up");
+        // same kernels and layout as sm_vesicle_radius: [R][4] head sums, [R][3] centres,
+        // [R] distance sums; the radii then replace the centres.  Every word read is
+        // written first by this call, so nothing is cleared.
+        double *sums = w, *com = w + 4 * (size_t)R, *dist = w + 7 * (size_t)R;
+        for (int what = 0; what < 4; what++) {
+            launch_head_terms(c->st, c->n, c->beads, c->reps, head_type, what, nullptr, c->ebead);
+            launch_seg_sum(c->st, c->segB, c->ebead, sums + what, 4);
+        }
+        launch_head_com(c->st, R, sums, com);
+        launch_head_terms(c->st, c->n, c->beads, c->reps, head_type, 4, com, c->ebead);
+        launch_seg_sum(c->st, c->segB, c->ebead, dist, 1);
+        launch_head_radius(c->st, R, sums, dist, com);
+        c->check_launch();
+        HIPCHK(hipMemcpyAsync(radius, com, sizeof(double) * (size_t)R, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
     });
 }
 

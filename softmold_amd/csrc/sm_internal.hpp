@@ -172,6 +172,16 @@ void launch_rescale(hipStream_t st, int n, Bead *beads, const DevRep *reps);
 void launch_fill_u32(hipStream_t st, uint32_t *p, size_t count, uint32_t value);
 void launch_set_step(hipStream_t st, DevGlobal *glob, long long step);
 
+// ---- fixed scratch of the entry points that allocate nothing (sm_noalloc.hip) ----------
+constexpr int kNoAllocMaxReplicas = 1024;
+constexpr size_t kNoAllocScratchDoubles = 8 * (size_t)kNoAllocMaxReplicas;   // 64 KiB per GPU
+// Looks up `device`'s copy of the scratch array once (that device must be current).
+hipError_t noalloc_scratch_init(int device);
+// The device's scratch address, or nullptr before noalloc_scratch_init(device).
+double *noalloc_scratch(int device);
+// radius[r] = dist[r] / sums[4r + 3], all device pointers
+void launch_head_radius(hipStream_t st, int nrep, const double *sums, const double *dist, double *radius);
+
 // ---- radix sort (sm_sort.hip) ---------------------------------------------------------
 struct SortBuffers {
     uint32_t *k0, *v0, *k1, *v1;
