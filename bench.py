@@ -39,7 +39,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/s at 1 GPU, plain vs batched FP32, and % of HBM roofline"
 HBM_PEAK_GBS = 8000.0
-VALU_FP32_PEAK_TFLOPS = 157.3
+VALU_FP32_PEAK_TFLOPS = 157.3          # published vector FP32 peak (counts packed FP32 FMA)
+VALU_FP32_UNPACKED_TFLOPS = 78.6       # the same without packed FP32, which this build disables
+                                       # (and the FP64 vector rate of the kernel's exact test)
 SEED, NLIPIDS, DENSITY = 1234, 80000, 3.45
 # HBM bytes per pair_force launch on this workload, from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE runs scaled by the same counters on a copy of known size (bench.py --calib-copy)
@@ -175,27 +177,31 @@ def bench_batched(steps: int, warmup: int, seq_steps: int) -> dict:
     log(f"batched: {len(systems)} replicas, {nb} beads, {1e3 * wall / steps:.3f} ms/step, "
         f"{batched:.4g} particle-steps/s, knobs {knobs}, MC acceptance {accepted.sum() / max(trials.sum(), 1):.3f}")
 
-    # the plain solver on each replica alone, in its default (eager) mode, where a step
-    # without a box move ends in the fused kick + next-first-half kernel: same per-step
-    # work, summed walls
-    seq_wall, seq_work = 0.0, 0
+    # the plain solver on each replica alone, at that replica's own best launch shape: the
+    # autotuner picks its pair chunk and the faster of captured graph and eager stepping
+    # (where a step without a box move ends in the fused kick + next-first-half kernel);
+    # the same steps and warmup as the batch, summed walls
+    seq_wall, seq_work, modes = 0.0, 0, {"graph": 0, "eager": 0}
     for d in systems:
         s = sm.Solver.from_system(d)
-        s.tune("graph", 0)
         s.init_forces()
-        s.run(1, 16)
+        s.autotune(32)
+        modes["graph" if s.tune_get("graph") else "eager"] += 1
+        s.run(1, warmup)
         s.sync()
         t0 = time.perf_counter()
-        s.run(17, seq_steps)
+        s.run(1 + warmup, seq_steps)
         s.sync()
         seq_wall += time.perf_counter() - t0
         seq_work += len(d["type"]) * seq_steps
         s.close()
     seq = seq_work / seq_wall
-    log(f"sequential plain solver: {seq:.4g} particle-steps/s ({seq_steps} steps per replica)")
+    log(f"sequential plain solver: {seq:.4g} particle-steps/s ({seq_steps} steps per replica, "
+        f"autotuned: {modes})")
     return {"value": batched, "unit": "particle-steps/s", "replicas": len(systems), "beads": nb,
             "steps": steps, "warmup": warmup, "ms_per_step": 1e3 * wall / steps,
             "sequential_value": seq, "sequential_steps_per_replica": seq_steps,
+            "sequential_modes": modes,
             "speedup_vs_sequential": batched / seq, "autotuned": knobs,
             "autotune_best_ms_per_step": tuned_ms,
             "workload": "64 liposome replicas (seeds 1000..1063), lipid counts log-uniform in "
@@ -252,7 +258,7 @@ def main():
         calib_copy()
         return
     if args.batched_only:
-        print(json.dumps({"batched": bench_batched(args.batched_steps, 100, 200)}), flush=True)
+        print(json.dumps({"batched": bench_batched(args.batched_steps, 100, args.batched_steps)}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the solver has no multi-GPU path "
@@ -301,6 +307,13 @@ def main():
     flops = n * (9.0 * cand + 14.0 * accepted)
     tflops = flops / (pair_ms * 1e-3) / 1e12
     log(f"N={n} occupied cells={occ} candidates/bead={cand:.1f} neighbours/bead={accepted:.1f}")
+    # the whole step against HBM: the per-bead algorithmic bytes of an unfused FP32 step
+    # (integrate + key 76, radix sort 16 per 8-bit pass, reorder 36, pair 28, bonded 40,
+    # scatter 16, second half + Langevin 36) and 8 B per occupied cell
+    bits = max(1, int(np.ceil(np.log2(float(np.prod(nc))))))
+    passes = (bits + 7) // 8
+    step_bytes = (232.0 + 16.0 * passes) * n + 8.0 * occ
+    step_gbs = step_bytes / (ms_per_step * 1e-3) / 1e9
 
     out = {
         "metric": METRIC,
@@ -319,16 +332,24 @@ def main():
         "config": {"workload": "plain solver, vesicle 80000 lipids = 240000 beads, box 400^3, "
                                "dt 0.02, T 3, gamma 1, tension 0",
                    "n_beads": n, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic(),
+        # the dominant kernel (the pair force, two thirds of the step) is bound by VALU
+        # issue: its algorithmic flops (9 per stencil candidate + 14 per neighbour, per bead)
+        # against the published FP32 vector peak, with its compulsory HBM bytes beside them
+        "roofline": {"bound": "valu", "achieved": tflops, "peak": VALU_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tflops / VALU_FP32_PEAK_TFLOPS, "traffic": traffic(),
                      "kernel": "pair_force", "kernel_avg_us": pair_ms * 1e3,
-                     "algorithmic_bytes_per_launch": algo_bytes,
-                     "valu_tflops": tflops, "valu_frac_fp32": tflops / VALU_FP32_PEAK_TFLOPS,
-                     "valu_issue_frac": valu_issue()},
+                     "algorithmic_flops_per_launch": flops,
+                     "frac_unpacked_fp32": tflops / VALU_FP32_UNPACKED_TFLOPS,
+                     "valu_issue_frac": valu_issue(),
+                     "hbm": {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": algo_bytes}},
+        "step_roofline": {"bound": "hbm", "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": step_gbs / HBM_PEAK_GBS, "algorithmic_bytes_per_step": step_bytes,
+                          "sort_passes": passes},
     }
     s.close()
     if not args.no_batched:
-        out["batched"] = bench_batched(args.batched_steps, 100, 200)
+        out["batched"] = bench_batched(args.batched_steps, 100, args.batched_steps)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(n)

@@ -818,6 +818,7 @@ struct orc_md {
     orc_mt *lstreams;   /* Langevin: MTRand(seed + t) per thread (langevin.h:98-107) */
     orc_mt mc;          /* MC box move: MTRand(seed) (MD.cpp:149) */
     int32_t trials, accepted;
+    uint64_t decisions;   /* accept bits of the trials so far, latest in bit 0 */
 };
 
 orc_md *orc_md_create(orc_sys *s, int32_t rng_mode, int32_t mt_streams)
@@ -882,7 +883,8 @@ static void mc_move(orc_md *m, int step)
     double D = dP / s->temp;
     D = exp(D);
     if (m->rng_mode == 0) u = orc_mt_rand53(&m->mc);
-    if (D >= u || -dP <= 0) {
+    const int acc = (D >= u || -dP <= 0);
+    if (acc) {
         for (int i = 0; i < s->n; i++) {
             s->pos[3 * (size_t)i] *= aS[0];
             s->pos[3 * (size_t)i + 1] *= aS[1];
@@ -891,6 +893,7 @@ static void mc_move(orc_md *m, int step)
         s->box[0] = size[0]; s->box[1] = size[1]; s->box[2] = size[2];
         m->accepted++;
     }
+    m->decisions = (m->decisions << 1) | (uint64_t)acc;
     m->trials++;
 }
 
@@ -911,6 +914,7 @@ int32_t orc_md_run(orc_md *m, int32_t start, int32_t end)
 }
 
 int32_t orc_md_trials(const orc_md *m) { return m->trials; }
+uint64_t orc_md_decisions(const orc_md *m) { return m->decisions; }
 
 /* The objective of the constant-fitting adjoint: init forces, steps 1..nsteps of the loop
  * above, J = mean over the steps of U_pair + U_chain taken after the kick and before the

@@ -98,6 +98,8 @@ void orc_md_destroy(orc_md *m);
 void orc_md_init_forces(orc_md *m);            /* MD.cpp:186-262 */
 int32_t orc_md_run(orc_md *m, int32_t start, int32_t end); /* MD.cpp:335-729; returns #accepted MC moves */
 int32_t orc_md_trials(const orc_md *m);
+/* accept (1) / reject (0) bits of the MC trials so far, the latest in bit 0 */
+uint64_t orc_md_decisions(const orc_md *m);
 double orc_md_objective(orc_md *m, int32_t nsteps);
 /* analysis/vesicleRadius.cpp:71-175, one frame */
 double orc_vesicle_radius(const double *pos, const int32_t *type, int32_t n, const double box[3], int32_t head);         /* mean U after each kick, steps 1..nsteps */

@@ -92,6 +92,7 @@ def _declare(L):
         "orc_md_init_forces": (None, [C.c_void_p]),
         "orc_md_run": (i32, [C.c_void_p, i32, i32]),
         "orc_md_trials": (i32, [C.c_void_p]),
+        "orc_md_decisions": (C.c_uint64, [C.c_void_p]),
         "orc_md_objective": (d, [C.c_void_p, i32]),
         "orc_md_measure": (None, [C.c_void_p, P(d), P(d)]),
     }
@@ -345,6 +346,10 @@ class MD:
         U, K = C.c_double(), C.c_double()
         lib().orc_md_measure(self.h, C.byref(U), C.byref(K))
         return U.value, K.value
+
+    def decisions(self) -> int:
+        """Accept (1) / reject (0) bits of the MC trials so far, the latest in bit 0."""
+        return int(lib().orc_md_decisions(self.h))
 
     def objective(self, nsteps: int) -> float:
         """Init forces, steps 1..nsteps; mean U_pair + U_chain after each kick."""

@@ -180,7 +180,8 @@ __device__ inline Cand load_cand(const Cand *p)
 }
 
 // One candidate of the force kernels: the exact FP64 test r^2 < rc^2 (MD.h:798-810), then
-// the force law (MD.h:811-848) in FP32.  With r0 = r2f * rsq(r2f) (exact as an FP64
+// the force law (MD.h:811-848) in FP32, its magnitude times the FP64 difference added to
+// FP64 sums.  With r0 = r2f * rsq(r2f) (exact as an FP64
 // product), r = r0 + corr is one Newton step against the FP64 residual r^2 - r0^2, and
 // m = C - r is formed as fma(-r2f, rsq, C) - corr, so it carries no FP32 rounding of r^2
 // or r0 (near r_m the force is C1 m with C1 ~ 400).  1/r is rsq(r2f) refined by one Newton
@@ -189,7 +190,7 @@ __device__ inline Cand load_cand(const Cand *p)
 // the branch r >= r_m selects (r_c, C4, C5) over (r_m, C1, C2), which is the reference's
 // row offset int(r/r_m)*3 clamped (MD.h:819).
 __device__ inline void force_term(const Bead &hb, const Cand &q, bool other, double cut2, const float4 *C8,
-                                  float &fx, float &fy, float &fz)
+                                  double &fx, double &fy, double &fz)
 {
     const double dxd = hb.x - q.x;
     const double dyd = hb.y - q.y;
@@ -198,7 +199,6 @@ __device__ inline void force_term(const Bead &hb, const Cand &q, bool other, dou
     // the type bound keeps the type's LDS read ahead of the branch, with the position
     if (r2 < cut2 && other && (unsigned)q.t < (unsigned)kMaxTypes) {
         const float4 A = C8[2 * q.t], B = C8[2 * q.t + 1];
-        const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
         const float r2f = (float)r2;
         const float ri0 = __builtin_amdgcn_rsqf(r2f);
         const double r0d = (double)r2f * (double)ri0;
@@ -209,9 +209,13 @@ __device__ inline void force_term(const Bead &hb, const Cand &q, bool other, dou
         const float c0 = outer ? A.w : A.x, c1 = outer ? B.x : A.y, c2 = outer ? B.y : A.z;
         const float m = fmaf(-r2f, ri0, c0) - corr;
         const float mag = ((c1 - c2 * m) * m) * ri;
-        fx += dx * mag;
-        fy += dy * mag;
-        fz += dz * mag;
+        // the pair's force, mag * (FP64 difference), accumulated in FP64: a bead sums ~60
+        // terms of up to a few hundred, so FP32 partial sums would round at 2^-24 of that
+        // each time, more than the terms' own FP32 error
+        const double magd = (double)mag;
+        fx = fma(dxd, magd, fx);
+        fy = fma(dyd, magd, fy);
+        fz = fma(dzd, magd, fz);
     }
 }
 
@@ -385,7 +389,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                 const int hslot = s0 + h0 + hi;
                 const uint32_t me = act ? sidx[s0 + h0 + hi] : 0u;
                 const float hfx = (float)(hb.x - ox0), hfy = (float)(hb.y - oy0), hfz = (float)(hb.z - oz0);
-                float fx = 0.f, fy = 0.f, fz = 0.f;
+                double fx = 0.0, fy = 0.0, fz = 0.0;
                 double accu = 0.0;
                 int cnt = 0;
                 for (int cb = 0; cb < total; cb += kMaxC) {
@@ -604,14 +608,14 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                     // a home's tpp lanes combined by a fixed-shape tree: at step s, lane sub
                     // adds lane sub + s when sub is a multiple of 2s
                     for (int st = 1; st < tpp; st <<= 1) {
-                        const float ox = __shfl(fx, lane + st), oy = __shfl(fy, lane + st), oz = __shfl(fz, lane + st);
+                        const double ox = __shfl(fx, lane + st), oy = __shfl(fy, lane + st), oz = __shfl(fz, lane + st);
                         if ((sub & (2 * st - 1)) == 0 && sub + st < tpp) {
                             fx += ox;
                             fy += oy;
                             fz += oz;
                         }
                     }
-                    if (act && sub == 0) out.force[me] = make_float4(fx, fy, fz, 0.f);
+                    if (act && sub == 0) out.force[me] = make_float4((float)fx, (float)fy, (float)fz, 0.f);
                 } else if (MODE == M_COUNT || kFC) {
                     for (int st = 1; st < tpp; st <<= 1) {
                         const int o = __shfl(cnt, lane + st);

@@ -41,7 +41,9 @@ THETA0 = np.array([-6.0, 200.0, 100.0, 100.0])
 STEP = np.array([6e-3, 0.2, 0.5, 0.5])   # central-difference steps
 
 
-def _oracle_J(d, theta, nsteps):
+def _oracle_J(d, theta, nsteps, decisions=False):
+    """The FP64 oracle's J along its own trajectory; with decisions=True also the accept
+    bits of its box-move trials."""
     import oracle as O
     from softmold_amd.adjoint import pair_rows
 
@@ -53,37 +55,50 @@ def _oracle_J(d, theta, nsteps):
     s = O.System(d["pos"], d["type"], d["vel"], d["box"], F, U, d["n_types"], d["chain"], cc,
                  cutoff=d["cutoff"], dt=d["dt"], gamma=d["gamma"], temp=d["temperature"],
                  delta_lxy=d["delta_lxy"], tension=d["tension"], seed=d["seed"])
-    return O.MD(s, rng_mode=1).objective(nsteps)
+    md = O.MD(s, rng_mode=1)
+    J = md.objective(nsteps)
+    return (J, md.decisions()) if decisions else J
 
 
-def _fd64(d, S, scale):
-    """Central differences of the FP64 oracle's J with steps STEP * scale."""
+def _fd64(d, S, scale, decisions=None):
+    """Central differences of the FP64 oracle's J with steps STEP * scale.  With
+    `decisions` (the accept bits at THETA0), every perturbed run must make the same box-move
+    decisions: the pathwise derivative holds them fixed, and a flipped decision would make
+    the difference jump."""
     fd = np.zeros(4)
     for p in range(4):
         h = STEP[p] * scale
         a, b = THETA0.copy(), THETA0.copy()
         a[p] += h
         b[p] -= h
-        fd[p] = (_oracle_J(d, a, S) - _oracle_J(d, b, S)) / (2 * h)
+        if decisions is None:
+            fd[p] = (_oracle_J(d, a, S) - _oracle_J(d, b, S)) / (2 * h)
+        else:
+            (ja, da), (jb, db) = _oracle_J(d, a, S, True), _oracle_J(d, b, S, True)
+            assert da == db == decisions, (p, bin(da), bin(db), bin(decisions))
+            fd[p] = (ja - jb) / (2 * h)
     return fd
 
 
 @pytest.mark.gpu
 def test_adjoint_gradient_matches_fp64_reference():
-    """40 steps: the trajectory is chaotic enough that central differences need small
-    steps to stay in the linear regime (at STEP they are off by up to 17 %, converging to
-    the adjoint as the step shrinks), so the FP64 oracle, free of FP32 noise, supplies
-    the reference gradient with steps STEP * 1e-4."""
+    """32 steps: the trajectory is chaotic enough that central differences need small
+    steps to stay in the linear regime (at STEP they are off by up to 17 % at 40 steps,
+    converging to the adjoint as the step shrinks), so the FP64 oracle, free of FP32 noise,
+    supplies the reference gradient with steps STEP * 1e-4.  The adjoint follows the
+    FP32-forward trajectory, which leaves the FP64 one at the chaotic rate: against the
+    FP64 reference gradient its largest component error was 4e-6 at 24 steps, 2e-5 at 32
+    and 8e-4 at 40."""
     if not gpu_available():
         pytest.skip("no GPU")
     from softmold_amd.adjoint import get_params, objective
 
     d = _system()
-    S, k = 40, 10
+    S, k = 32, 10
     s = sm.Solver.from_system(d)
     assert np.allclose(get_params(s, PAIR), THETA0)
     before = s.download()
-    print("case: adjoint B400 40 steps, checkpoint 10", flush=True)
+    print("case: adjoint B400 32 steps, checkpoint 10", flush=True)
     J, g = objective(s, PAIR, S, k)
     after = s.download()
     for x, y in zip(before, after):
@@ -98,6 +113,93 @@ def test_adjoint_gradient_matches_fp64_reference():
     assert J == pytest.approx(J64, rel=1e-6)
     np.testing.assert_allclose(g, fd64, rtol=1e-3)
     np.testing.assert_allclose(g, gref, rtol=1e-3)
+
+
+@pytest.mark.gpu
+def test_adjoint_with_box_moves_matches_fp64_differences():
+    """Constant-tension box moves on (deltaLXY 0.01, tension 1, a trial every 8 steps):
+    accepted moves rescale the positions, which the adjoint carries through x -> S x with
+    the decisions held fixed.  32 steps hold 4 trials; the FP64 oracle's central
+    differences (steps STEP * 1e-4) make the same decisions at every perturbed theta."""
+    if not gpu_available():
+        pytest.skip("no GPU")
+    from softmold_amd.adjoint import objective
+
+    d = sm.Mpd.bilayer(1234, 400, 3.45).system()
+    d["delta_lxy"], d["tension"] = 0.01, 1.0
+    S, k = 32, 10
+    s = sm.Solver.from_system(d)
+    print("case: adjoint B400 with box moves, 32 steps, checkpoint 10", flush=True)
+    J, g = objective(s, PAIR, S, k)
+    J64, dec = _oracle_J(d, THETA0, S, True)
+    assert dec != 0, "no trial accepted: the rescaling branch would not be exercised"
+    fd64 = _fd64(d, S, 1e-4, decisions=dec)
+    print(f"decisions {dec:04b} J gpu {J:.10g} oracle {J64:.10g}\nadjoint {g}\nFP64 central differences {fd64}")
+    assert J == pytest.approx(J64, rel=1e-6)
+    np.testing.assert_allclose(g, fd64, rtol=1e-3)
+
+
+@pytest.mark.gpu
+def test_adjoint_on_the_4000_lipid_bilayer():
+    """The adjoint's configuration: the 4000-lipid bilayer (12,000 beads) as the generator
+    writes it (box moves on, deltaLXY 0.01), 200 steps, checkpoint every 20.
+
+    Central differences of J have a linear regime only over a short horizon: measured with
+    the FP64 oracle on this system (Umin steps 6e-4 ... 6e-13), they agree to 1e-5 up to 40
+    steps, drift by 10 % at 80 and grow without bound from 120 steps (to 1e11 at 200): the
+    trajectory's sensitivity grows as e^(lambda t) with lambda ~ 6 per unit time, and the
+    FP32-forward trajectory the adjoint follows leaves the FP64 one at that rate (largest
+    component error against the FP64 reference gradient 3e-6 at 24 steps, 4e-4 at 32, 1e-3
+    at 40).  So the 200-step run is timed and checked for J, finiteness and checkpoint
+    invariance, and the gradient is checked over 24 steps with the same checkpoint interval
+    (segments of 20 and 4 steps): with the box moves against FP64 central differences
+    that make the same decisions, and without them also against the FP64 CPU reference
+    gradient."""
+    if not gpu_available():
+        pytest.skip("no GPU")
+    import time
+
+    import adjoint_ref as AR
+    from softmold_amd.adjoint import objective
+
+    d = sm.Mpd.bilayer(1234, 4000, 3.45).system()
+    assert len(d["type"]) == 12000 and d["delta_lxy"] == 0.01
+    s = sm.Solver.from_system(d)
+    print("case: adjoint B4000 200 steps, checkpoint 20", flush=True)
+    s.sync()
+    t0 = time.perf_counter()
+    J, g = objective(s, PAIR, 200, 20)
+    wall = time.perf_counter() - t0
+    J1, g1 = objective(s, PAIR, 200, 200)
+    J64 = _oracle_J(d, THETA0, 200)
+    print(f"200 steps: adjoint wall {wall:.3f} s, J gpu {J:.10g} oracle {J64:.10g} "
+          f"(relative {abs(J - J64) / abs(J64):.2e}), gradient {g}")
+    assert np.all(np.isfinite(g))
+    assert J == J1 and np.array_equal(g, g1)
+    # the FP32 trajectory decorrelates from the FP64 one in the run's second half (same
+    # noise), so the time-averaged energies agree statistically: the potential energy of
+    # 12,000 beads fluctuates by ~1e-3 of its mean
+    assert abs(J - J64) <= 1e-3 * abs(J64)
+
+    S, k = 24, 20
+    print("case: adjoint B4000 with box moves, 24 steps, checkpoint 20", flush=True)
+    Jm, gm = objective(s, PAIR, S, k)
+    J64m, dec = _oracle_J(d, THETA0, S, True)
+    fdm = _fd64(d, S, 1e-4, decisions=dec)
+    print(f"decisions {dec:03b} J gpu {Jm:.10g} oracle {J64m:.10g}\nadjoint {gm}\nFP64 central differences {fdm}")
+    assert Jm == pytest.approx(J64m, rel=1e-6)
+    np.testing.assert_allclose(gm, fdm, rtol=1e-3)
+
+    d0 = dict(d)
+    d0["delta_lxy"] = 0.0
+    s0 = sm.Solver.from_system(d0)
+    print("case: adjoint B4000 24 steps, checkpoint 20", flush=True)
+    J0, g0 = objective(s0, PAIR, S, k)
+    fd0 = _fd64(d0, S, 1e-4)
+    gref = AR.adjoint_gradient(d0, THETA0, PAIR, S)
+    print(f"J gpu {J0:.10g}\nadjoint {g0}\nFP64 central differences {fd0}\nFP64 reference gradient {gref}")
+    np.testing.assert_allclose(g0, fd0, rtol=1e-3)
+    np.testing.assert_allclose(g0, gref, rtol=1e-3)
 
 
 @pytest.mark.gpu
