@@ -3,8 +3,10 @@
 Runs only in the build container, where the reference sources exist: it builds
 oracle/_ref/ with `make -C oracle ref` (the reference's own MD.cpp, generators and the
 ref_dump harness compiled from the sources where they lie) and records what those
-binaries print and write.  The fixtures are data only: generator checksums, step-0 FP64
-forces/keys/neighbour counts/energies, and short single-thread MD trajectories.
+binaries print and write.  The fixtures are data only: generator checksums (the named
+systems, the 1M-lipid bilayer and the 64 liposomes of the replica sweep), step-0 FP64
+forces/keys/neighbour counts/energies, short single-thread MD trajectories, and the bond
+and bend statistics MD writes for a system with BOND and BEND records.
 
 usage: python tests/golden/make_golden.py
 """
@@ -32,7 +34,44 @@ GEN = {
     "B4000": ["bilayer", "1234", "4000", "3.45", "0", "0", "0", "0"],
     "L80000": ["liposome", "1234", "80000", "3.45"],
     "L2000s7": ["liposome", "7", "2000", "3.45"],
+    "B1M": ["bilayer", "1234", "1000000", "3.45", "0", "0", "0", "0"],   # 999,698 lipids
 }
+
+
+def sweep_gen():
+    """The replica sweep's liposomes: seed 1000 + r with the sweep's lipid counts."""
+    import sys
+
+    sys.path.insert(0, REPO)
+    from softmold_amd.sweep import sweep_counts
+
+    return {f"S{r:02d}": ["liposome", str(1000 + r), str(int(n)), "3.45"] for r, n in enumerate(sweep_counts())}
+
+
+def bond_stats_input(work):
+    """The reference's B400 bilayer with two more molecules: a BOND molecule (l0 1.5, k 20)
+    between 80 random head pairs and a BEND molecule (c 0.5, k 10) over 40 random head
+    triples.  Heads lie across the whole 10.8-wide box, so many differences are imaged."""
+    import sys
+
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+
+    src = os.path.join(work, "B400.mpd")
+    heads = np.flatnonzero(O.read_mpd(src)["type"] == 2)
+    rng = np.random.default_rng(11)
+    b = rng.choice(heads, size=(80, 2))
+    b = b[b[:, 0] != b[:, 1]]
+    t = rng.choice(heads, size=(40, 3))
+    t = t[(t[:, 0] != t[:, 1]) & (t[:, 1] != t[:, 2]) & (t[:, 0] != t[:, 2])]
+    text = open(src).read().replace("nMolecules 1 ", "nMolecules 3 ", 1)
+    assert "nMolecules 3 " in text
+    # the new molecules follow the CHAIN record, before the keys written after it
+    chain = "molecule \n8\t 1\n 0.7  100  1  100 \n 0  392  3 \n"
+    assert chain in text
+    extra = f"6\t {len(b)}\n 1.5 20 \n" + "".join(f" {i} {j}\n" for i, j in b)
+    extra += f"7\t {len(t)}\n 0.5 10 \n" + "".join(f" {i} {j} {k}\n" for i, j, k in t)
+    return text.replace(chain, chain + extra, 1)
 STEP0 = ["L500", "B400"]
 
 
@@ -76,10 +115,16 @@ def main():
     work = tempfile.mkdtemp(prefix="smgold_")
     try:
         sha, step0, traj = {}, {}, {}
-        for name, argv in GEN.items():
+        gens = dict(GEN)
+        gens.update(sweep_gen())
+        for name, argv in gens.items():
             run([os.path.join(REF, argv[0]), name] + argv[1:], work)
-            sha[name] = {"argv": argv,
-                         "sha256": hashlib.sha256(open(os.path.join(work, name + ".mpd"), "rb").read()).hexdigest()}
+            path = os.path.join(work, name + ".mpd")
+            sha[name] = {"argv": argv, "sha256": hashlib.sha256(open(path, "rb").read()).hexdigest()}
+            if name == "B1M" or name.startswith("S"):
+                with open(path) as f:
+                    sha[name]["n_particles"] = int(next(ln.split()[1] for ln in f if ln.startswith("nParticles")))
+                os.remove(path)
         # step-0 quantities from the reference's own classes
         for name in STEP0 + ["L5000"]:
             r = run([os.path.join(REF, "ref_dump"), name, name] + [repr(x) for x in SCALE], work)
@@ -118,6 +163,18 @@ def main():
             if os.path.exists(hist):
                 rec["resizeHist"] = read_dat(hist)
             traj[tname] = rec
+        # bond and bend statistics of MD's step-0 measurement (dataExtraction.h:861-930,
+        # 1009-1027): the first line of lBond_ and bend_ is written before the loop
+        d = os.path.join(work, "bstats")
+        os.makedirs(d)
+        with open(os.path.join(d, "BS.mpd"), "w") as f:
+            f.write(bond_stats_input(work))
+        edit_mpd(os.path.join(d, "BS.mpd"), finalTime=repr(0.0), storeInterval=repr(1.0), measureInterval=repr(0.02))
+        rec = {"steps": 0, "input_mpd": open(os.path.join(d, "BS.mpd")).read()}
+        run([os.path.join(REF, "MD"), "BS"], d)
+        rec["lBond"] = read_dat(os.path.join(d, "lBond_BS.dat"))
+        rec["bend"] = read_dat(os.path.join(d, "bend_BS.dat"))
+        traj["BS"] = rec
 
         with open(os.path.join(HERE, "gen_sha256.json"), "w") as f:
             json.dump(sha, f, indent=1, sort_keys=True)

@@ -65,6 +65,27 @@ def _deformed(d, seed=7, amp=0.15):
 
 # ---- oracle (CPU) --------------------------------------------------------------------
 
+def test_oracle_bond_stats_match_reference_md_output(golden_traj, tmp_path):
+    """oracle.bond_stats against the reference MD's own step-0 lBond_ and bend_ lines
+    (dataExtraction.h:861-930,1009-1027) for the B400 bilayer with a BOND molecule over 80
+    random head pairs and a BEND molecule over 40 random head triples; heads span the
+    10.8-wide box, so many differences take the >= / <= image.  MD prints 6 significant
+    digits."""
+    path = tmp_path / "BS.mpd"
+    path.write_bytes(golden_traj["BS/input_mpd"].tobytes())
+    m = O.read_mpd(str(path))
+    mols = {mol["type"]: np.array(mol["bonds"]) for mol in m["molecule"]}
+    bonds, bends = mols[6], mols[7]
+    assert len(bonds) == 80 and len(bends) == 40
+    got = O.bond_stats(m["pos"], m["size"], bonds, bends)
+    lb, bd = golden_traj["BS/lBond"], golden_traj["BS/bend"]
+    assert lb[0, 0] == 0.0 and bd[0, 0] == 0.0
+    np.testing.assert_allclose(got, [lb[0, 1], bd[0, 1], bd[0, 2], bd[0, 3]], rtol=6e-6, atol=0)
+    # the imaging matters at this size: without it the mean bond length is far larger
+    raw = np.linalg.norm(m["pos"][bonds[:, 0]] - m["pos"][bonds[:, 1]], axis=1).mean()
+    assert raw > 1.05 * got[0]
+
+
 def test_oracle_records_equal_chains():
     d = sm.Mpd.liposome(5, 300, 3.45).system()
     pos = _deformed(d)

@@ -30,14 +30,44 @@ def _sha(path):
     return hashlib.sha256(open(path, "rb").read()).hexdigest()
 
 
-@pytest.mark.parametrize("name", ["L500", "B400", "L5000", "B4000", "L2000s7", "L80000"])
-def test_generators_match_reference_bytes(name, tmp_path):
-    rec = json.load(open(os.path.join(GOLDEN, "gen_sha256.json")))[name]
+def _generate_and_hash(rec, path):
     kind, seed, n, rho = rec["argv"][0], int(rec["argv"][1]), int(rec["argv"][2]), float(rec["argv"][3])
     m = sm.Mpd.liposome(seed, n, rho) if kind == "liposome" else sm.Mpd.bilayer(seed, n, rho)
-    out = str(tmp_path / name)
-    m.write(out)
-    assert _sha(out + ".mpd") == rec["sha256"]
+    nbeads = m["nParticles"]
+    m.write(path)
+    m.close()
+    h = _sha(path + ".mpd")
+    os.remove(path + ".mpd")
+    return h, nbeads
+
+
+@pytest.mark.parametrize("name", ["L500", "B400", "L5000", "B4000", "L2000s7", "L80000", "B1M"])
+def test_generators_match_reference_bytes(name, tmp_path):
+    """Byte-identical .mpd files against the reference generators' own output; B1M is the
+    1M-lipid flat bilayer, whose lattice caps it at 999,698 lipids (lipidModels.h:1822-1828)."""
+    rec = json.load(open(os.path.join(GOLDEN, "gen_sha256.json")))[name]
+    h, nbeads = _generate_and_hash(rec, str(tmp_path / name))
+    assert h == rec["sha256"]
+    if name == "B1M":
+        assert nbeads == rec["n_particles"] == 2999094
+
+
+def test_replica_sweep_inputs_match_reference_bytes(tmp_path):
+    """The 64 liposomes of the replica sweep (seeds 1000..1063, the sweep's lipid counts)
+    are byte-identical to the reference generator's files."""
+    from softmold_amd.sweep import sweep_counts
+
+    recs = json.load(open(os.path.join(GOLDEN, "gen_sha256.json")))
+    counts = sweep_counts()
+    total = 0
+    for r, n in enumerate(counts):
+        rec = recs[f"S{r:02d}"]
+        assert rec["argv"] == ["liposome", str(1000 + r), str(int(n)), "3.45"]
+        h, nbeads = _generate_and_hash(rec, str(tmp_path / f"S{r:02d}"))
+        assert h == rec["sha256"], r
+        assert nbeads == rec["n_particles"]
+        total += nbeads
+    assert total == 2805567
 
 
 def test_mpd_round_trip_is_byte_identical(golden_traj, tmp_path):
