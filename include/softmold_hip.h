@@ -187,6 +187,15 @@ int sm_autotune(sm_ctx *ctx, int32_t steps, double *best_ms_per_step);
  * No counterpart in the reference, whose constants are fixed per run; the callers are
  * constant-fitting loops. */
 int sm_set_params(sm_ctx *ctx, int32_t type_a, int32_t type_b, const double *theta);
+/* dU/dtheta at the current state, 4 per replica, FP64: d(U_pair + U_chain)/dUmin and /dUmax
+ * of the type pair (a, b), then /dk_bond and /dk_bend of the CHAIN sets.  U is linear in the
+ * laradjiRevaleePC row entries with r_m and r_c fixed (potentials/laradjiRevalee.h:124-152,
+ * Potential MD.h:895-930) and in the bond and bend stiffnesses (harmonicP MD.h:422-431,
+ * bendP MD.h:769-789), so these are the energy sums over derivative tables.  Generic
+ * BOND / BEND / BALL records carry their own constants and contribute nothing.  No
+ * counterpart in the reference; the callers are equilibrium-response estimators (the
+ * derivative of an ensemble average by its covariance with dU/dtheta). */
+int sm_param_derivative(sm_ctx *ctx, int32_t type_a, int32_t type_b, double *out);
 int sm_get_params(sm_ctx *ctx, int32_t type_a, int32_t type_b, double *theta);
 /* From the context's current positions and velocities: initial forces, then steps
  * 1..nsteps with the counter RNG (box moves as configured), J = mean over the steps of

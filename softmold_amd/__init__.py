@@ -57,7 +57,7 @@ EXPORTS = [
     "sm_second_half", "sm_step_first", "sm_step_finish", "sm_run", "sm_mc_stats", "sm_mc_hist",
     "sm_measure", "sm_vesicle_radius", "sm_vesicle_radius_noalloc", "sm_bond_stats", "sm_compute_forces", "sm_cell_keys", "sm_pair_count", "sm_force_pair_count", "sm_pair_list",
     "sm_potential_delta", "sm_tune", "sm_tune_get", "sm_autotune", "sm_set_params", "sm_get_params",
-    "sm_adjoint", "sm_profile_enable",
+    "sm_adjoint", "sm_param_derivative", "sm_profile_enable",
     "sm_profile_read", "sm_mpd_read", "sm_mpd_write", "sm_mpd_free", "sm_mpd_system",
     "sm_mpd_get", "sm_mpd_set", "sm_mpd_set_state", "sm_gen_liposome", "sm_gen_bilayer",
 ]
@@ -115,6 +115,7 @@ def _declare(L):
         "sm_set_params": (C.c_int, [vp, i32, i32, P(d)]),
         "sm_get_params": (C.c_int, [vp, i32, i32, P(d)]),
         "sm_adjoint": (C.c_int, [vp, i32, i32, i32, i32, P(d), P(d)]),
+        "sm_param_derivative": (C.c_int, [vp, i32, i32, P(d)]),
         "sm_profile_enable": (C.c_int, [vp, i32]),
         "sm_profile_read": (C.c_int, [vp, i32, P(C.c_char_p), P(d), P(i64), P(i32)]),
         "sm_mpd_read": (C.c_int, [C.c_char_p, P(vp)]),
@@ -408,6 +409,19 @@ class Solver:
         out = np.zeros(self.nrep)
         fn = lib().sm_vesicle_radius_noalloc if noalloc else lib().sm_vesicle_radius
         _check(fn(self.h, int(head_type), _dp(out)))
+        return self._one(out)
+
+    def box(self):
+        """Box lengths per replica (3 each)."""
+        box = np.zeros((self.nrep, 3))
+        _check(lib().sm_download_state(self.h, None, None, None, _dp(box)))
+        return self._one(box)
+
+    def param_derivative(self, pair):
+        """d(U_pair + U_chain)/d(Umin, Umax of `pair`, k_bond, k_bend) per replica, FP64, at
+        the current state."""
+        out = np.zeros((self.nrep, 4))
+        _check(lib().sm_param_derivative(self.h, int(pair[0]), int(pair[1]), _dp(out)))
         return self._one(out)
 
     def bond_stats(self):

@@ -9,7 +9,8 @@ recomputes the trajectory from checkpoints every `checkpoint` steps.
 
 Area per lipid changes only through the Monte-Carlo box moves, whose accept/reject
 decisions are piecewise constant in theta: its pathwise gradient is zero, which is what the
-adjoint (decisions held fixed) and small central differences both give.
+adjoint (decisions held fixed) and small central differences both give.  The gradient of
+its ensemble average comes from the fluctuation formula in softmold_amd.response.
 """
 from __future__ import annotations
 

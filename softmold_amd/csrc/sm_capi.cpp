@@ -151,6 +151,13 @@ struct sm_ctx {
     std::vector<double> hfconst, huconst;
     std::vector<double4> hcconst;
     double *adjF = nullptr, *adjDF = nullptr, *adjDU = nullptr;   // FP64 F table, d/dtheta tables
+    // sm_param_derivative: the U table's d/dUmin and d/dUmax forms of one type pair, their
+    // culling tables, and the CHAIN constant sets with (k_bond, k_bend) = (1, 0) and (0, 1)
+    double *pdU = nullptr;
+    float *pdTab = nullptr;
+    double4 *pdC = nullptr;
+    int pd_a = -1, pd_b = -1;
+    size_t pd_tab = 0;
     double *ebead = nullptr, *echain = nullptr, *dres = nullptr;   // dres: meas [R][3], then dU [R][2]
     SegTiles segB{}, segC{};
     int *dcounts = nullptr, *dpairs = nullptr;
@@ -188,7 +195,7 @@ struct sm_ctx {
         dfree(reps); dfree(glob); dfree(fconstF); dfree(uconstD); dfree(fconstD); dfree(chainPerBead); dfree(chainList);
         dfree(gptr); dfree(gent); dfree(gconstD);
         dfree(cconstD); dfree(ebead); dfree(echain); dfree(dres); dfree(dcounts); dfree(dpairs); dfree(dnp);
-        dfree(adjF); dfree(adjDF); dfree(adjDU);
+        dfree(adjF); dfree(adjDF); dfree(adjDU); dfree(pdU); dfree(pdTab); dfree(pdC);
         dfree(segB.tile); dfree(segB.rep_first); dfree(segB.tilesum);
         dfree(segC.tile); dfree(segC.rep_first); dfree(segC.tilesum);
         if (st) (void)hipStreamDestroy(st);
@@ -1505,6 +1512,59 @@ void upload_adjoint_tables(sm_ctx *c, int a, int b)
     HIPCHK(hipMemcpyAsync(c->adjDU, dU.data(), sizeof(double) * 2 * nt, hipMemcpyHostToDevice, c->st));
 }
 
+// The tables sm_param_derivative runs the energy kernels with.  U is linear in each
+// laradjiRevaleePC row's entries {r_m, C1, C2, r_c, C4, C5} with r_m and r_c fixed
+// (laradjiRevalee.h:124-152), and those entries are linear in Umin and Umax, so the FP64
+// potential gather run on a table whose (a, b) and (b, a) rows hold the entries'
+// derivatives (every row keeping its r_m and r_c, every other coefficient 0) sums
+// dU_pair/dUmin or dU_pair/dUmax.  harmonicP and bendP are linear in k_bond and k_bend
+// (MD.h:422-431,769-789), so the CHAIN energy with (k_bond, k_bend) = (1, 0) or (0, 1) is
+// dU_chain/dk_bond or dU_chain/dk_bend.  Depends only on r_m, r_c, l0 and the cos-theta
+// term, which the fitting parameters leave alone; rebuilt when the pair changes.
+void upload_param_derivative_tables(sm_ctx *c, int a, int b)
+{
+    if (c->pdU && c->pd_a == a && c->pd_b == b) return;
+    const int nT = c->P.nT;
+    const size_t nt = 6 * (size_t)nT * nT;
+    std::vector<double> dU(2 * nt, 0.0);
+    for (size_t row = 0; row < nt; row += 6)
+        for (int p = 0; p < 2; p++) {
+            dU[p * nt + row] = c->huconst[row];
+            dU[p * nt + row + 3] = c->huconst[row + 3];
+        }
+    for (int pass = 0; pass < 2; pass++) {
+        const size_t row = 6 * (size_t)(pass ? b * nT + a : a * nT + b);
+        const double rm = c->huconst[row], rc = c->huconst[row + 3];
+        const double w = rc - rm;
+        double *Umin = &dU[row], *Umax = &dU[nt + row];
+        Umin[1] = -1.0 / (rm * rm);
+        Umin[2] = 1.0;
+        Umin[4] = 3.0 / (w * w);
+        Umin[5] = 2.0 / (w * w * w);
+        Umax[1] = 1.0 / (rm * rm);
+    }
+    std::vector<double> dUmin(dU.begin(), dU.begin() + nt), dUmax(dU.begin() + nt, dU.end());
+    const std::vector<float> t0 = pair_table(c->hfconst, dUmin, nT, c->P.cutoff);
+    const std::vector<float> t1 = pair_table(c->hfconst, dUmax, nT, c->P.cutoff);
+    std::vector<double4> cc;
+    for (const double4 &k : c->hcconst) cc.push_back(make_double4(k.x, 1.0, k.z, 0.0));
+    for (const double4 &k : c->hcconst) cc.push_back(make_double4(k.x, 0.0, k.z, 1.0));
+    if (!c->pdU) {
+        c->pdU = dalloc<double>(2 * nt);
+        c->pd_tab = t0.size();
+        c->pdTab = dalloc<float>(2 * c->pd_tab);
+        c->pdC = dalloc<double4>(cc.size());
+    }
+    HIPCHK(hipMemcpyAsync(c->pdU, dU.data(), sizeof(double) * 2 * nt, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->pdTab, t0.data(), sizeof(float) * c->pd_tab, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->pdTab + c->pd_tab, t1.data(), sizeof(float) * c->pd_tab, hipMemcpyHostToDevice, c->st));
+    if (!cc.empty())
+        HIPCHK(hipMemcpyAsync(c->pdC, cc.data(), sizeof(double4) * cc.size(), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->pd_a = a;
+    c->pd_b = b;
+}
+
 // device copy of the evolving state
 struct StateCopy {
     Bead *beads = nullptr;
@@ -1568,6 +1628,39 @@ int sm_get_params(sm_ctx *c, int32_t type_a, int32_t type_b, double *theta)
         theta[1] = F[1] * rm * rm / 2.0 + umin;
         theta[2] = c->hcconst.empty() ? 0.0 : c->hcconst[0].y;
         theta[3] = c->hcconst.empty() ? 0.0 : c->hcconst[0].w;
+    });
+}
+
+int sm_param_derivative(sm_ctx *c, int32_t type_a, int32_t type_b, double *out)
+{
+    return guarded([&] {
+        if (!c || !out) invalid("null argument");
+        check_pair(c, type_a, type_b);
+        upload_param_derivative_tables(c, type_a, type_b);
+        c->build(0, kStepKeep);
+        const int R = c->nrep;
+        const size_t nt = 6 * (size_t)c->P.nT * c->P.nT;
+        double *w = c->dres;   // 5 words per replica: the four derivatives, then free
+        for (int p = 0; p < 2; p++) {
+            launch_pair_gather(c->st, c->P, c->chunk, GATHER_U, c->sbeads, c->skey(), c->sidx(), c->cstart, c->cend,
+                               c->reps, c->pdTab + p * c->pd_tab, c->pdU + p * nt, c->ebead, nullptr, nullptr, nullptr,
+                               0);
+            launch_seg_sum(c->st, c->segB, c->ebead, w + p, 5);
+        }
+        const size_t ncs = c->hcconst.size();
+        // the measurement words are free between calls (each measurement copies its own out)
+        if (c->nchains > 0)
+            for (int p = 0; p < 2; p++) {
+                launch_chain_energy(c->st, c->nchains, c->chainList, c->pdC + p * ncs, c->beads, c->reps, 0, c->echain);
+                launch_seg_sum(c->st, c->segC, c->echain, w + 2 + p, 5);
+            }
+        c->check_launch();
+        std::vector<double> h((size_t)5 * R);
+        HIPCHK(hipMemcpyAsync(h.data(), w, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        for (int r = 0; r < R; r++)
+            for (int p = 0; p < 4; p++) out[4 * (size_t)r + p] = (p >= 2 && c->nchains == 0) ? 0.0 : h[5 * r + p];
+        c->check_flags();
     });
 }
 
