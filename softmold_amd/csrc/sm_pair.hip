@@ -114,13 +114,6 @@ __device__ inline double sq3(double x, double y, double z)
     return (x * x + y * y) + z * z;
 }
 
-// a*s - b*s with both products rounded, as the reference scales both positions before
-// differencing them (cellOpt.h:1098-1105)
-__device__ inline double scaled_diff(double a, double b, double s)
-{
-#pragma clang fp contract(off)
-    return a * s - b * s;
-}
 
 
 // sqrt of a positive, normal double: v_rsq_f64 and the two Goldschmidt corrections the
@@ -138,6 +131,34 @@ __device__ inline double sqrt_pos(double x)
     d = fma(-r, r, x);
     r = fma(d, h, r);
     return x > 0.0 ? r : 0.0;
+}
+
+// sqrt_pos's sequence returning r and 1/r (twice the refined half-reciprocal)
+__device__ inline void sqrt_inv_pos(double x, double &r, double &ri)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double rr = x * y, h = 0.5 * y;
+    const double e = fma(-h, rr, 0.5);
+    rr = fma(rr, e, rr);
+    h = fma(h, e, h);
+    double d = fma(-rr, rr, x);
+    rr = fma(d, h, rr);
+    d = fma(-rr, rr, x);
+    rr = fma(d, h, rr);
+    r = rr;
+    ri = 2.0 * h;
+}
+
+// Potential<double> (MD.h:895-930) at a distance dr already taken, both branches evaluated
+// and one selected
+__device__ inline double pair_u_r(double dr, const double *C)
+{
+#pragma clang fp contract(off)
+    const double a = C[0] - dr;
+    const double inner = C[1] * a * a + C[2];
+    const double b = C[3] - dr;
+    const double outer = b * b * (C[4] - b * C[5]);
+    return dr <= C[0] ? inner : outer;
 }
 
 // Potential<double> (MD.h:895-930) with both branches evaluated and one selected, so a
@@ -324,6 +345,12 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
             const double csx = R.cs[0], csy = R.cs[1], csz = R.cs[2];
             const float hcx = (float)(0.5 * csx), hcy = (float)(0.5 * csy), hcz = (float)(0.5 * csz);
             const double ax = R.scale[0], ay = R.scale[1], az = R.scale[2];
+            // dU: s^2 - 1 per axis as (s - 1)(s + 1), exact in its leading bits since s - 1 is
+            // exact for a trial scale within a factor 2 of 1
+            const double s2x = (ax - 1.0) * (ax + 1.0), s2y = (ay - 1.0) * (ay + 1.0), s2z = (az - 1.0) * (az + 1.0);
+            // the scaled distance by a series in delta = (r'^2 - r^2) / r^2, a weighted mean of
+            // the s_k^2 - 1 (wave-uniform; larger trial scales take two square roots)
+            const bool dseries = fmax(fabs(s2x), fmax(fabs(s2y), fabs(s2z))) <= 1e-2;
             // energy gathers: a candidate farther than (rc + margin) / min(1, scale) from the
             // home quarter-cell is beyond rc both before and after the trial scaling
             const double smin = fmin(1.0, fmin(ax, fmin(ay, az)));
@@ -550,24 +577,48 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         const uint8_t *lst = mapW + quad * kMaxC;
                         const int nl = quad == 0 ? nq[0] : (quad == 1 ? nq[1] : (quad == 2 ? nq[2] : nq[3]));
                         const double *Ch = sCd + 6 * hb.type * nT;
-                        for (int jj = sub; jj < nl; jj += tpp) {
-                            const int i = lst[jj];
-                            const int j = cb + i;
-                            const Cand q = Cw[i];
-                            const int c = q.c;
-                            if (j == self || (c == 13 && j < self)) continue;
-                            const double px = q.x, py = q.y, pz = q.z;   // staged imaged
+                        // one owned pair's share: U, or U(old) - U(scaled).  Under the trial
+                        // scale every coordinate difference scales by s_k, so the scaled r^2 is
+                        // r^2 + sum (s_k^2 - 1) d_k^2 from the differences already at hand; it
+                        // differs from differencing the scaled positions (cellOpt.h:1098-1105)
+                        // by a rounding, where the potential is continuous
+                        auto term = [&](const Cand &q, int j) -> double {
+                            if (j == self || (q.c == 13 && j < self)) return 0.0;
                             const double *C = Ch + 6 * q.t;
-                            const double r2 = sq3(hb.x - px, hb.y - py, hb.z - pz);
-                            if (MODE == M_U) {
-                                accu += r2 < cut2 ? pair_u_any(r2, C) : 0.0;
-                            } else {
-                                const double r2n = sq3(scaled_diff(hb.x, px, ax), scaled_diff(hb.y, py, ay),
-                                                       scaled_diff(hb.z, pz, az));
-                                const double oldP = r2 < cut2 ? pair_u_any(r2, C) : 0.0;
-                                const double newP = r2n < cut2 ? pair_u_any(r2n, C) : 0.0;
-                                accu += (oldP - newP);
+                            const double dx = hb.x - q.x, dy = hb.y - q.y, dz = hb.z - q.z;   // staged imaged
+                            const double r2 = sq3(dx, dy, dz);
+                            if (MODE == M_U) return r2 < cut2 ? pair_u_any(r2, C) : 0.0;
+                            const double dr2 = fma(s2x, dx * dx, fma(s2y, dy * dy, s2z * (dz * dz)));
+                            const double r2n = r2 + dr2;
+                            const bool in0 = r2 < cut2, in1 = r2n < cut2;
+                            if (!(in0 || in1)) return 0.0;
+                            if (!dseries) return (in0 ? pair_u_any(r2, C) : 0.0) - (in1 ? pair_u_any(r2n, C) : 0.0);
+                            // r' - r = r (sqrt(1 + delta) - 1) by the binomial series through
+                            // delta^5: with |delta| <= 1e-2 the rest is below 5e-12 of dr
+                            double r, ri;
+                            sqrt_inv_pos(r2, r, ri);
+                            const double del = dr2 * (ri * ri);
+                            const double dr =
+                                (r * del) *
+                                fma(del, fma(del, fma(del, fma(del, 0.02734375, -0.0390625), 0.0625), -0.125), 0.5);
+                            const double rn = r + dr;
+                            const bool innerO = r <= C[0], innerN = rn <= C[0];
+                            if (in0 && in1 && innerO == innerN) {
+                                // both distances on one polynomial branch: U(r) - U(r + dr) in
+                                // closed form, with no cancellation between the two potentials
+                                const double a = C[0] - r;
+                                const double uin = C[1] * dr * fma(2.0, a, -dr);
+                                const double b = C[3] - r;
+                                const double bn = b - dr;
+                                const double uout = dr * (C[4] * fma(2.0, b, -dr) - C[5] * fma(3.0 * b, bn, dr * dr));
+                                return innerO ? uin : uout;
                             }
+                            // a branch or the cutoff lies between the two distances
+                            return (in0 ? pair_u_r(r, C) : 0.0) - (in1 ? pair_u_r(rn, C) : 0.0);
+                        };
+                        for (int jj = sub; jj < nl; jj += tpp) {
+                            const int i0 = lst[jj];
+                            accu += term(Cw[i0], cb + i0);
                         }
                     } else if (!kF && act) {
                         // neighbour counts and pair lists see both sides of every pair
