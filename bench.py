@@ -2,6 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched]
     python bench.py --calib-copy      # plain 512 MiB device copies, for HBM counter scaling
+    python bench.py --bilayer-only    # the flat bilayer of 999,698 lipids (2,999,094 beads)
 
 Workload (`value`): the 80,000-lipid vesicle (240,000 beads) from the liposome generator
 (seed 1234, areal density 3.45, box 400^3, dt 0.02, T 3, gamma 1, tension 0 so no box
@@ -209,6 +210,33 @@ def bench_batched(steps: int, warmup: int, seq_steps: int) -> dict:
                         "temperature 3*linspace(0.9,1.1,8), deltaLXY 0.01, MC box move every 8 steps"}
 
 
+def bench_bilayer(steps: int, warmup: int) -> dict:
+    """The reference generator's flat bilayer `bilayer 1234 1000000 3.45` (999,698 lipids,
+    2,999,094 beads, box 538.38^2 x 40, deltaLXY 0.01: a box move every 8 steps)."""
+    import torch
+
+    import softmold_amd as sm
+
+    d = sm.Mpd.bilayer(1234, 1000000, 3.45).system()
+    n = len(d["type"])
+    s = sm.Solver.from_system(d)
+    s.init_forces()
+    s.run(0, 1 + warmup)
+    s.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.run(1 + warmup, steps)
+    s.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    trials, accepted = s.mc_stats()
+    s.close()
+    return {"value": n * steps / wall, "unit": "particle-steps/s", "beads": n, "steps": steps, "warmup": warmup,
+            "ms_per_step": 1e3 * wall / steps, "mc_trials": trials, "mc_accepted": accepted,
+            "workload": "flat bilayer, reference generator bilayer 1234 1000000 3.45 (999,698 lipids), "
+                        "deltaLXY 0.01, box move every 8 steps"}
+
+
 def calib_copy():
     """Five copies of a 512 MiB buffer: the reference dispatch for scaling HBM counters."""
     import torch
@@ -253,9 +281,13 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=1000)
     ap.add_argument("--calib-copy", action="store_true")
     ap.add_argument("--batched-only", action="store_true", help="only the replica sweep (for profiling it)")
+    ap.add_argument("--bilayer-only", action="store_true", help="only the 1M-lipid flat bilayer (for profiling it)")
     args = ap.parse_args()
     if args.calib_copy:
         calib_copy()
+        return
+    if args.bilayer_only:
+        print(json.dumps({"bilayer": bench_bilayer(args.steps, args.warmup)}), flush=True)
         return
     if args.batched_only:
         print(json.dumps({"batched": bench_batched(args.batched_steps, 100, args.batched_steps)}), flush=True)
