@@ -44,6 +44,11 @@ typedef struct sm_system {
     int32_t n_chain_blocks;     /* CHAIN records: (start, nChains, length) (molecules.h:38-40) */
     const int32_t *chain;       /* 3 per block */
     const double *chain_const;  /* 4 per block: bond length, k_bond, cos-theta term, k_bend */
+    /* Two-body tables, row t1*n_types + t2.  A pair takes the row of the particle the
+     * reference's half-shell traversal makes first (MD.h:811-819, cellOpt.h:712-926),
+     * which only matters where the (a,b) and (b,a) rows differ.  Rows with r_m below
+     * cutoff/2 are refused (SM_ERR_UNSUPPORTED): the reference's branch offset
+     * int(r/r_m)*3 (MD.h:819) would leave the row. */
     const double *fconst;       /* twoBodyFconst, 6*n_types^2 (potentials/laradjiRevalee.h:84-112) */
     const double *uconst;       /* twoBodyUconst, 6*n_types^2 (potentials/laradjiRevalee.h:124-152) */
     double box[3];

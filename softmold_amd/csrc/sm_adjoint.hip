@@ -131,6 +131,17 @@ __device__ inline double sq3(double x, double y, double z)
     return (x * x + y * y) + z * z;
 }
 
+// The reference's forward half shell (cellOpt.h:715) as a mask over the stencil index
+// (dx+1) + 3(dy+1) + 9(dz+1).
+__host__ __device__ constexpr uint32_t forward_cells()
+{
+    const int nb[13][3] = {{1, 0, 0}, {0, 1, 0}, {1, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1},
+                           {-1, 0, 1}, {0, -1, 1}, {-1, 1, 1}, {1, -1, 1}, {-1, -1, 1}, {1, -1, 0}};
+    uint32_t m = 0;
+    for (int j = 0; j < 13; j++) m |= 1u << ((nb[j][0] + 1) + 3 * (nb[j][1] + 1) + 9 * (nb[j][2] + 1));
+    return m;
+}
+
 __global__ __launch_bounds__(kBlock) void k_adj_pair(int n, int nT, double cut2, const Bead *__restrict__ sb,
                                                      const uint32_t *__restrict__ skey,
                                                      const uint32_t *__restrict__ sidx,
@@ -174,7 +185,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_pair(int n, int nT, double cut2,
             const double r2 = sq3(dx, dy, dz);
             if (!(r2 < cut2)) continue;
             const double r = sqrt(r2);
-            const size_t row = 6 * ((size_t)hb.type * nT + q.type);
+            // the row of the particle the reference's traversal makes first: across cells
+            // the one whose forward half shell (cellOpt.h:715) holds the other's cell,
+            // within a cell the lower index (cellOpt.h:572-585,780-795)
+            constexpr uint32_t fwd = forward_cells();
+            const bool first = c == 13 ? me < sidx[j] : ((fwd >> c) & 1u) != 0;
+            const size_t row = 6 * (first ? (size_t)hb.type * nT + q.type : (size_t)q.type * nT + hb.type);
             const double *C = fc + row;
             const int o = (r >= C[0]) ? 3 : 0;
             const double m = C[o] - r;

@@ -56,8 +56,9 @@ const char *kClassName[K_NCLASS] = {"integrate_first+key", "radix_sort", "reorde
 // constants padded to 8 floats {r_m, C1, C2, r_c, C4, C5, 0, 0}, then the culling reach
 // r^2 of each candidate type for the force (from those FP32 constants) and for the FP64
 // energy gathers (from the FP64 potential constants), kMaxTypes floats each.  A candidate
-// type's reach is the largest over the home types of r_c when the pair has an outer branch
-// (C4 or C5 nonzero) and r_m otherwise, capped at the cutoff, plus 1e-4.
+// type's reach is the largest over the other types, and over both orders of the pair, of
+// r_c when the row has an outer branch (C4 or C5 nonzero) and r_m otherwise, capped at the
+// cutoff, plus 1e-4.
 std::vector<float> pair_table(const std::vector<double> &hf, const std::vector<double> &hu, int nT, double cutoff)
 {
     const size_t rows = (size_t)nT * nT;
@@ -67,20 +68,34 @@ std::vector<float> pair_table(const std::vector<double> &hf, const std::vector<d
     const double rcap = std::sqrt(cutoff * cutoff);
     for (int t = 0; t < nT; t++) {
         double rF = 0.0, rE = 0.0;
-        for (int h = 0; h < nT; h++) {
-            const size_t row = (size_t)h * nT + t;
-            const float *f = &tab[8 * row];
-            const double fm = f[0], fc = f[3], f4 = f[4], f5 = f[5];
-            rF = std::fmax(rF, (f4 != 0.0 || f5 != 0.0) ? fc : fm);
-            const double *u = &hu[6 * row];
-            rE = std::fmax(rE, (u[4] != 0.0 || u[5] != 0.0) ? u[3] : u[0]);
-        }
+        for (int h = 0; h < nT; h++)
+            for (const size_t row : {(size_t)h * nT + t, (size_t)t * nT + h}) {
+                const float *f = &tab[8 * row];
+                const double fm = f[0], fc = f[3], f4 = f[4], f5 = f[5];
+                rF = std::fmax(rF, (f4 != 0.0 || f5 != 0.0) ? fc : fm);
+                const double *u = &hu[6 * row];
+                rE = std::fmax(rE, (u[4] != 0.0 || u[5] != 0.0) ? u[3] : u[0]);
+            }
         rF = std::fmin(rF, rcap) + 1e-4;
         rE = std::fmin(rE, rcap) + 1e-4;
         tab[8 * rows + t] = (float)(rF * rF);
         tab[8 * rows + kMaxTypes + t] = (float)(rE * rE);
     }
     return tab;
+}
+
+// Whether some (a, b) row of the force or potential table differs from its (b, a) row.  The
+// reference evaluates each pair once, with the row of the particle its traversal makes
+// first (MD.h:811-819, cellOpt.h:712-926); the pair kernels then pick that row per pair.
+bool tables_asymmetric(const std::vector<double> &hf, const std::vector<double> &hu, int nT)
+{
+    for (int a = 0; a < nT; a++)
+        for (int b = a + 1; b < nT; b++)
+            for (int k = 0; k < 6; k++)
+                if (hf[6 * (a * nT + b) + k] != hf[6 * (b * nT + a) + k] ||
+                    hu[6 * (a * nT + b) + k] != hu[6 * (b * nT + a) + k])
+                    return true;
+    return false;
 }
 
 // k_first's step argument: set the device step counter, advance it by one, or leave it
@@ -549,14 +564,12 @@ void validate(const sm_system *s)
                 invalid("particle " + std::to_string(i) + " position is out of bounds");
         }
     }
-    // The full-shell gather evaluates (t1,t2) from one side and (t2,t1) from the other;
-    // the reference's half shell evaluates one order only, so asymmetric tables differ.
-    for (int a = 0; a < nT; a++)
-        for (int b = 0; b < nT; b++)
-            for (int k = 0; k < 6; k++)
-                if (s->fconst[6 * (a * nT + b) + k] != s->fconst[6 * (b * nT + a) + k] ||
-                    s->uconst[6 * (a * nT + b) + k] != s->uconst[6 * (b * nT + a) + k])
-                    unsupported("non-symmetric two-body constant tables are not supported");
+    // the reference's force takes the branch at row offset int(r / r_m) * 3 (MD.h:819),
+    // which stays inside the row for every r below the cutoff only while 2 r_m >= cutoff
+    for (int r = 0; r < nT * nT; r++)
+        if (!(2.0 * s->fconst[6 * r] >= s->cutoff))
+            unsupported("two-body row " + std::to_string(r / nT) + "," + std::to_string(r % nT) +
+                        ": r_m below half the cutoff (the reference's branch offset int(r/r_m)*3 leaves the row)");
     if (s->n_chain_blocks < 0) invalid("negative chain block count");
     if (s->n_chain_blocks > 0 && (!s->chain || !s->chain_const)) invalid("chain blocks without data");
     if (s->n_bonds < 0 || s->n_bends < 0 || s->n_balls < 0) invalid("negative BOND / BEND / BALL record count");
@@ -747,6 +760,7 @@ sm_ctx *create(const sm_system *s, int nrep)
         const size_t nconst = 6 * (size_t)nT * nT;
         c->hfconst.assign(s[0].fconst, s[0].fconst + nconst);
         c->huconst.assign(s[0].uconst, s[0].uconst + nconst);
+        c->P.asym = tables_asymmetric(c->hfconst, c->huconst, nT) ? 1 : 0;
         const std::vector<float> fF = pair_table(c->hfconst, c->huconst, nT, s[0].cutoff);
         c->hcconst = ccD;
 
@@ -1471,6 +1485,12 @@ void set_pair_rows(sm_ctx *c, int a, int b, double umin, double umax)
 void upload_constants(sm_ctx *c)
 {
     const size_t nt = c->hfconst.size();
+    // a captured graph holds the pair kernels of the tables' symmetry
+    const int asym = tables_asymmetric(c->hfconst, c->huconst, c->P.nT) ? 1 : 0;
+    if (asym != c->P.asym) {
+        c->P.asym = asym;
+        c->drop_graph();
+    }
     const std::vector<float> fF = pair_table(c->hfconst, c->huconst, c->P.nT, c->P.cutoff);
     HIPCHK(hipMemcpyAsync(c->fconstF, fF.data(), sizeof(float) * fF.size(), hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->uconstD, c->huconst.data(), sizeof(double) * nt, hipMemcpyHostToDevice, c->st));

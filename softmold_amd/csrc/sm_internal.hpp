@@ -81,6 +81,9 @@ struct Params {
     int nT;
     double cutoff;
     double dt;
+    // the two-body tables' (a, b) and (b, a) rows differ somewhere: the pair kernels then
+    // take each pair's row from the particle the reference's traversal makes first
+    int asym;
 };
 
 // Counter-RNG stream tags (word 3 of the Philox counter).

@@ -206,10 +206,35 @@ __device__ inline Cand load_cand(const Cand *p)
 // product), r = r0 + corr is one Newton step against the FP64 residual r^2 - r0^2, and
 // m = C - r is formed as fma(-r2f, rsq, C) - corr, so it carries no FP32 rounding of r^2
 // or r0 (near r_m the force is C1 m with C1 ~ 400).  1/r is rsq(r2f) refined by one Newton
-// step against r.  C8 holds the home type's constant rows padded to 8 floats, one row
-// {r_m, C1, C2, r_c, C4, C5, 0, 0} per candidate type, read as two float4 in one LDS trip;
-// the branch r >= r_m selects (r_c, C4, C5) over (r_m, C1, C2), which is the reference's
-// row offset int(r/r_m)*3 clamped (MD.h:819).
+// step against r.  row is the pair's row of the constant table padded to 8 floats,
+// {r_m, C1, C2, r_c, C4, C5, 0, 0}, read as two float4 in one LDS trip; the branch
+// r >= r_m selects (r_c, C4, C5) over (r_m, C1, C2), which is the reference's row offset
+// int(r/r_m)*3 clamped (MD.h:819).
+__device__ inline void force_law(double dxd, double dyd, double dzd, double r2, const float4 *row, double &fx,
+                                 double &fy, double &fz)
+{
+    const float4 A = row[0], B = row[1];
+    const float r2f = (float)r2;
+    const float ri0 = __builtin_amdgcn_rsqf(r2f);
+    const double r0d = (double)r2f * (double)ri0;
+    const float corr = (float)fma(-r0d, r0d, r2) * (0.5f * ri0);
+    const float r = (float)r0d + corr;
+    const float ri = ri0 * (2.0f - r * ri0);
+    const bool outer = r >= A.x;
+    const float c0 = outer ? A.w : A.x, c1 = outer ? B.x : A.y, c2 = outer ? B.y : A.z;
+    const float m = fmaf(-r2f, ri0, c0) - corr;
+    const float mag = ((c1 - c2 * m) * m) * ri;
+    // the pair's force, mag * (FP64 difference), accumulated in FP64: a bead sums ~60
+    // terms of up to a few hundred, so FP32 partial sums would round at 2^-24 of that
+    // each time, more than the terms' own FP32 error
+    const double magd = (double)mag;
+    fx = fma(dxd, magd, fx);
+    fy = fma(dyd, magd, fy);
+    fz = fma(dzd, magd, fz);
+}
+
+// force_term over a two-body table whose (a, b) and (b, a) rows agree: C8 holds the home
+// type's rows, one per candidate type.
 __device__ inline void force_term(const Bead &hb, const Cand &q, bool other, double cut2, const float4 *C8,
                                   double &fx, double &fy, double &fz)
 {
@@ -218,25 +243,37 @@ __device__ inline void force_term(const Bead &hb, const Cand &q, bool other, dou
     const double dzd = hb.z - q.z;
     const double r2 = sq3(dxd, dyd, dzd);
     // the type bound keeps the type's LDS read ahead of the branch, with the position
-    if (r2 < cut2 && other && (unsigned)q.t < (unsigned)kMaxTypes) {
-        const float4 A = C8[2 * q.t], B = C8[2 * q.t + 1];
-        const float r2f = (float)r2;
-        const float ri0 = __builtin_amdgcn_rsqf(r2f);
-        const double r0d = (double)r2f * (double)ri0;
-        const float corr = (float)fma(-r0d, r0d, r2) * (0.5f * ri0);
-        const float r = (float)r0d + corr;
-        const float ri = ri0 * (2.0f - r * ri0);
-        const bool outer = r >= A.x;
-        const float c0 = outer ? A.w : A.x, c1 = outer ? B.x : A.y, c2 = outer ? B.y : A.z;
-        const float m = fmaf(-r2f, ri0, c0) - corr;
-        const float mag = ((c1 - c2 * m) * m) * ri;
-        // the pair's force, mag * (FP64 difference), accumulated in FP64: a bead sums ~60
-        // terms of up to a few hundred, so FP32 partial sums would round at 2^-24 of that
-        // each time, more than the terms' own FP32 error
-        const double magd = (double)mag;
-        fx = fma(dxd, magd, fx);
-        fy = fma(dyd, magd, fy);
-        fz = fma(dzd, magd, fz);
+    if (r2 < cut2 && other && (unsigned)q.t < (unsigned)kMaxTypes)
+        force_law(dxd, dyd, dzd, r2, C8 + 2 * q.t, fx, fy, fz);
+}
+
+// Which side of a pair the reference's half-shell traversal makes its first particle p1,
+// whose type is the row of the two-body table (MD.h:811-819): across cells the particle
+// of the cell whose forward half shell holds the other's cell (cellOpt.h:715,800-854);
+// within a cell the lower particle index, the cell's list running in descending index
+// (cellOpt.h:572-585,780-795).  c is the candidate's stencil cell seen from the home
+// particle, me and other the two original indices.
+__device__ inline bool home_first(int c, uint32_t me, uint32_t other)
+{
+    return c == 13 ? me < other : ((forward_mask() >> c) & 1u) != 0;
+}
+
+// force_term for two-body tables whose (a, b) and (b, a) rows differ: the candidate's
+// type carries its stencil cell above bit 8, its slot gives its original index through
+// sidx, and the row is the first particle's.  The same law on either side of a pair takes
+// the same row, so the two sides' forces are equal and opposite as in the reference.
+__device__ inline void force_term_asym(const Bead &hb, const Cand &q, bool other, double cut2, const float4 *C8all,
+                                       int nT, uint32_t me, const uint32_t *__restrict__ sidx, double &fx,
+                                       double &fy, double &fz)
+{
+    const double dxd = hb.x - q.x;
+    const double dyd = hb.y - q.y;
+    const double dzd = hb.z - q.z;
+    const double r2 = sq3(dxd, dyd, dzd);
+    if (r2 < cut2 && other) {
+        const int tq = q.t & 0xff, c = q.t >> 8;
+        const int row = home_first(c, me, c == 13 ? sidx[q.c] : 0u) ? hb.type * nT + tq : tq * nT + hb.type;
+        force_law(dxd, dyd, dzd, r2, C8all + 2 * row, fx, fy, fz);
     }
 }
 
@@ -244,11 +281,17 @@ __device__ inline void force_term(const Bead &hb, const Cand &q, bool other, dou
 // then the pair counted when its force term is not identically zero -- r^2 < r_m^2, or a row
 // with a nonzero outer coefficient (Umin != 0, laradjiRevalee.h:97-110).  Cd holds the home
 // type's FP64 force rows {r_m, C1, C2, r_c, C4, C5}.
-__device__ inline void count_term(const Bead &hb, const Cand &q, bool other, double cut2, const double *Cd, int &cnt)
+template <bool ASYM>
+__device__ inline void count_term(const Bead &hb, const Cand &q, bool other, double cut2, const double *Cd,
+                                  const double *Call, int nT, uint32_t me, const uint32_t *__restrict__ sidx, int &cnt)
 {
     const double r2 = sq3(hb.x - q.x, hb.y - q.y, hb.z - q.z);
-    if (r2 < cut2 && other && (unsigned)q.t < (unsigned)kMaxTypes) {
+    if (r2 < cut2 && other && (ASYM || (unsigned)q.t < (unsigned)kMaxTypes)) {
         const double *C = Cd + 6 * q.t;
+        if (ASYM) {
+            const int tq = q.t & 0xff, c = q.t >> 8;
+            C = Call + 6 * (home_first(c, me, c == 13 ? sidx[q.c] : 0u) ? hb.type * nT + tq : tq * nT + hb.type);
+        }
         if (C[4] != 0.0 || C[5] != 0.0 || r2 < C[0] * C[0]) cnt++;
     }
 }
@@ -262,7 +305,9 @@ struct Out {
     long long cap;
 };
 
-template <int MODE>
+// ASYM: the two-body table's (a, b) and (b, a) rows may differ, and each pair takes the row
+// of the particle the reference's traversal makes first (home_first).
+template <int MODE, bool ASYM>
 __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(4))) void k_cells(
     int n, int nT, int chunk, int nblocks, double cut2, float cut2pre, const Bead *__restrict__ sb,
     const uint32_t *__restrict__ skey, const uint32_t *__restrict__ sidx, const int *__restrict__ cstart,
@@ -485,9 +530,10 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         cd.z = (kF || kE || kFC) ? iz : qb[it].z;
                         // force: the candidate's sorted slot (the self test reads it with the
                         // type, in the same 16-byte load); energy gathers: its stencil cell;
-                        // count and list gathers: the slot, with the stencil cell above the
-                        // type's bits (types are below kMaxTypes <= 256)
-                        constexpr bool kSlotCell = MODE == M_COUNT || MODE == M_LIST;
+                        // count and list gathers, and the force over an asymmetric table: the
+                        // slot, with the stencil cell above the type's bits (types are below
+                        // kMaxTypes <= 256)
+                        constexpr bool kSlotCell = MODE == M_COUNT || MODE == M_LIST || (ASYM && (kF || kFC));
                         cd.t = kSlotCell ? (qb[it].type | (c << 8)) : qb[it].type;
                         cd.c = (kF || kFC || kSlotCell) ? src[it] : c;
                         Cw[lane + it * kWave] = cd;
@@ -528,8 +574,11 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                             const Cand q0 = load_cand(Cw + i0);
                             const Cand q1 = load_cand(Cw + i1);
                             if (kFC) {
-                                count_term(hb, q0, q0.c != hslot, cut2, Cd, cnt);
-                                count_term(hb, q1, q1.c != hslot, cut2, Cd, cnt);
+                                count_term<ASYM>(hb, q0, q0.c != hslot, cut2, Cd, sCd, nT, me, sidx, cnt);
+                                count_term<ASYM>(hb, q1, q1.c != hslot, cut2, Cd, sCd, nT, me, sidx, cnt);
+                            } else if (ASYM) {
+                                force_term_asym(hb, q0, q0.c != hslot, cut2, sC8, nT, me, sidx, fx, fy, fz);
+                                force_term_asym(hb, q1, q1.c != hslot, cut2, sC8, nT, me, sidx, fx, fy, fz);
                             } else {
                                 force_term(hb, q0, q0.c != hslot, cut2, C8, fx, fy, fz);
                                 force_term(hb, q1, q1.c != hslot, cut2, C8, fx, fy, fz);
@@ -540,7 +589,9 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         if (j < nl) {
                             const Cand q0 = load_cand(Cw + i0);
                             if (kFC)
-                                count_term(hb, q0, q0.c != hslot, cut2, Cd, cnt);
+                                count_term<ASYM>(hb, q0, q0.c != hslot, cut2, Cd, sCd, nT, me, sidx, cnt);
+                            else if (ASYM)
+                                force_term_asym(hb, q0, q0.c != hslot, cut2, sC8, nT, me, sidx, fx, fy, fz);
                             else
                                 force_term(hb, q0, q0.c != hslot, cut2, C8, fx, fy, fz);
                         }
@@ -567,7 +618,13 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                             for (int o = 1; o < kWave; o <<= 1) most = max(most, __shfl_xor(most, o));
                             // phase 2: exact FP64 test and the force law on the hits
                             for (int kk = 0; kk < most; kk++)
-                                if (kk < nhit) force_term(hb, load_cand(Cw + lst[kWave * kk]), true, cut2, C8, fx, fy, fz);
+                                if (kk < nhit) {
+                                    const Cand q = load_cand(Cw + lst[kWave * kk]);
+                                    if (ASYM)
+                                        force_term_asym(hb, q, true, cut2, sC8, nT, me, sidx, fx, fy, fz);
+                                    else
+                                        force_term(hb, q, true, cut2, C8, fx, fy, fz);
+                                }
                         }
                     } else if (kE && act) {
                         // each pair once, from the side that owns it: only the home cell and
@@ -584,7 +641,12 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         // by a rounding, where the potential is continuous
                         auto term = [&](const Cand &q, int j) -> double {
                             if (j == self || (q.c == 13 && j < self)) return 0.0;
+                            // the staged forward half shell's pairs have the home particle
+                            // first; a home-cell pair the lower original index
                             const double *C = Ch + 6 * q.t;
+                            if constexpr (ASYM) {
+                                if (q.c == 13 && sidx[s0 + (j - selfOff)] < me) C = sCd + 6 * (q.t * nT + hb.type);
+                            }
                             const double dx = hb.x - q.x, dy = hb.y - q.y, dz = hb.z - q.z;   // staged imaged
                             const double r2 = sq3(dx, dy, dz);
                             if (MODE == M_U) return r2 < cut2 ? pair_u_any(r2, C) : 0.0;
@@ -686,7 +748,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     }
 }
 
-template <int MODE>
+template <int MODE, bool ASYM = false>
 void launch_cells(hipStream_t st, const Params &P, int chunk, const Bead *sbeads, const uint32_t *skey,
                   const uint32_t *sidx, const int *cstart, const int *cend, const DevRep *reps,
                   const float *fconst, const double *uconst, const Out &out)
@@ -701,7 +763,7 @@ void launch_cells(hipStream_t st, const Params &P, int chunk, const Bead *sbeads
     // the two-phase kernel's FP32 screening radius (the culled modes take theirs per
     // candidate type from the constant tables, in the kernel)
     const float cut2pre = (float)(cut2 + slack);
-    hipLaunchKernelGGL(k_cells<MODE>, dim3(nb), dim3(kWaves * kWave), 0, st, P.n, P.nT, chunk, nb, cut2, cut2pre,
+    hipLaunchKernelGGL((k_cells<MODE, ASYM>), dim3(nb), dim3(kWaves * kWave), 0, st, P.n, P.nT, chunk, nb, cut2, cut2pre,
                        sbeads, skey, sidx, cstart, cend, reps, fconst, uconst, out);
 }
 
@@ -718,8 +780,12 @@ void launch_pair_force(hipStream_t st, const Params &P, int chunk, int variant, 
                        const DevRep *reps, const float *fconst, float4 *fpair)
 {
     Out o{fpair, nullptr, nullptr, nullptr, nullptr, 0};
-    if (variant == 1)
+    if (variant == 1 && P.asym)
+        launch_cells<M_FORCE, true>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, fconst, nullptr, o);
+    else if (variant == 1)
         launch_cells<M_FORCE>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, fconst, nullptr, o);
+    else if (P.asym)
+        launch_cells<M_FORCE1, true>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, fconst, nullptr, o);
     else
         launch_cells<M_FORCE1>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, fconst, nullptr, o);
 }
@@ -731,13 +797,26 @@ void launch_pair_gather(hipStream_t st, const Params &P, int chunk, int mode, co
 {
     Out o{nullptr, energy, counts, pairs, npairs, cap};
     switch (mode) {
-    case GATHER_U: launch_cells<M_U>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, tab, uconst, o); break;
-    case GATHER_DU: launch_cells<M_DU>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, tab, uconst, o); break;
+    case GATHER_U:
+        if (P.asym)
+            launch_cells<M_U, true>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, tab, uconst, o);
+        else
+            launch_cells<M_U>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, tab, uconst, o);
+        break;
+    case GATHER_DU:
+        if (P.asym)
+            launch_cells<M_DU, true>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, tab, uconst, o);
+        else
+            launch_cells<M_DU>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, tab, uconst, o);
+        break;
     case GATHER_COUNT:
         launch_cells<M_COUNT>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, tab, uconst, o);
         break;
     case GATHER_FCOUNT:
-        launch_cells<M_FCOUNT>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, tab, uconst, o);
+        if (P.asym)
+            launch_cells<M_FCOUNT, true>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, tab, uconst, o);
+        else
+            launch_cells<M_FCOUNT>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, tab, uconst, o);
         break;
     default: launch_cells<M_LIST>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, tab, uconst, o); break;
     }

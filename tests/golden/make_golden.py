@@ -5,7 +5,8 @@ oracle/_ref/ with `make -C oracle ref` (the reference's own MD.cpp, generators a
 ref_dump harness compiled from the sources where they lie) and records what those
 binaries print and write.  The fixtures are data only: generator checksums (the named
 systems, the 1M-lipid bilayer and the 64 liposomes of the replica sweep), step-0 FP64
-forces/keys/neighbour counts/energies, short single-thread MD trajectories, and the bond
+forces/keys/neighbour counts/energies, short single-thread MD trajectories (also of two
+systems whose two-body tables differ between the (a, b) and (b, a) rows), and the bond
 and bend statistics MD writes for a system with BOND and BEND records.
 
 usage: python tests/golden/make_golden.py
@@ -72,7 +73,29 @@ def bond_stats_input(work):
     extra = f"6\t {len(b)}\n 1.5 20 \n" + "".join(f" {i} {j}\n" for i, j in b)
     extra += f"7\t {len(t)}\n 0.5 10 \n" + "".join(f" {i} {j} {k}\n" for i, j, k in t)
     return text.replace(chain, chain + extra, 1)
-STEP0 = ["L500", "B400"]
+# Two-body rows of an asymmetric table: (2, 3) keeps an attractive outer branch, (3, 2) is
+# purely repulsive with a longer r_m, both from laradjiRevaleeFC / laradjiRevaleePC
+# (potentials/laradjiRevalee.h:84-152) with r_c = 2.  r_m stays at or above r_c / 2: the
+# force's branch offset int(r / r_m) * 3 (MD.h:819) is then 0 or 3 for every r < r_c.
+ASYM_ROWS = {(2, 3): (1.0, -4.0, 120.0), (3, 2): (1.1, 0.0, 260.0)}   # (r_m, Umin, Umax)
+
+
+def asym_tables(text, n_types=6, rc=2.0):
+    """The .mpd text with the ASYM_ROWS rows of twoBodyFconst and twoBodyUconst replaced."""
+    lines = text.split("\n")
+    for key in ("twoBodyFconst", "twoBodyUconst"):
+        at = next(i for i, ln in enumerate(lines) if ln.split() and ln.split()[0] == key)
+        for (a, b), (rm, umin, umax) in ASYM_ROWS.items():
+            w = rc - rm
+            if key == "twoBodyFconst":
+                row = [rm, 2.0 * (umax - umin) / (rm * rm), 0.0, rc, 6.0 * umin / (w * w), 6.0 * umin / (w * w * w)]
+            else:
+                row = [rm, (umax - umin) / (rm * rm), umin, rc, 3.0 * umin / (w * w), 2.0 * umin / (w * w * w)]
+            lines[at + 1 + a * n_types + b] = " " + " ".join(repr(float(x)) for x in row)
+    return "\n".join(lines)
+
+
+STEP0 = ["L500", "B400", "A500", "AB400"]
 
 
 def run(cmd, cwd, env=None):
@@ -125,6 +148,12 @@ def main():
                 with open(path) as f:
                     sha[name]["n_particles"] = int(next(ln.split()[1] for ln in f if ln.startswith("nParticles")))
                 os.remove(path)
+        # the same systems with asymmetric two-body tables
+        for src, name in (("L500", "A500"), ("B400", "AB400")):
+            with open(os.path.join(work, src + ".mpd")) as f:
+                text = asym_tables(f.read())
+            with open(os.path.join(work, name + ".mpd"), "w") as f:
+                f.write(text)
         # step-0 quantities from the reference's own classes
         for name in STEP0 + ["L5000"]:
             r = run([os.path.join(REF, "ref_dump"), name, name] + [repr(x) for x in SCALE], work)
@@ -143,6 +172,8 @@ def main():
         cases = {
             "L500": dict(steps=20, edits={}),
             "B400t": dict(steps=40, src="B400", edits={"tension": 1.0}),
+            "A500": dict(steps=20, edits={}),
+            "AB400t": dict(steps=40, src="AB400", edits={"tension": 1.0}),
         }
         for tname, c in cases.items():
             src = c.get("src", tname)

@@ -21,6 +21,12 @@ def single_thread():
     yield
 
 
+# step-0 fixture name -> the trajectory fixture whose input holds the same system
+SRC = {"L500": "L500", "B400": "B400t", "A500": "A500", "AB400": "AB400t"}
+# A500 and AB400 are L500 and B400 with two-body tables whose (2, 3) and (3, 2) rows differ
+STEP0 = ["L500", "B400", "A500", "AB400"]
+
+
 def _system(golden_traj, name):
     d = tempfile.mkdtemp()
     path = os.path.join(d, "in.mpd")
@@ -85,11 +91,10 @@ def test_mc_uniform_moments():
 # ---------------------------------------------------------------------------------------
 # cells and the pair set
 # ---------------------------------------------------------------------------------------
-@pytest.mark.parametrize("name", ["L500", "B400"])
+@pytest.mark.parametrize("name", STEP0)
 def test_cell_keys_match_reference(golden_step0, golden_traj, name):
     meta, arr = golden_step0
-    src = "B400t" if name == "B400" else name
-    s, _ = _system(golden_traj, src)
+    s, _ = _system(golden_traj, SRC[name])
     keys = s.cell_keys()
     np.testing.assert_array_equal(keys, arr[f"{name}/keys"])
     nc, cs = s.cell_grid()
@@ -108,10 +113,10 @@ def _brute_pairs(pos, box):
     return set(zip(i.tolist(), j.tolist()))
 
 
-@pytest.mark.parametrize("name", ["L500", "B400"])
+@pytest.mark.parametrize("name", STEP0)
 def test_pair_counts_match_reference_force_count(golden_step0, golden_traj, name):
     _, arr = golden_step0
-    s, _ = _system(golden_traj, "B400t" if name == "B400" else name)
+    s, _ = _system(golden_traj, SRC[name])
     np.testing.assert_array_equal(s.pair_count(), arr[f"{name}/counts"])
 
 
@@ -148,19 +153,19 @@ def test_pair_list_equals_brute_force(golden_traj, name):
 # ---------------------------------------------------------------------------------------
 # forces and energies
 # ---------------------------------------------------------------------------------------
-@pytest.mark.parametrize("name", ["L500", "B400"])
+@pytest.mark.parametrize("name", STEP0)
 def test_step0_forces_bit_exact_vs_reference(golden_step0, golden_traj, name):
     _, arr = golden_step0
-    s, _ = _system(golden_traj, "B400t" if name == "B400" else name)
+    s, _ = _system(golden_traj, SRC[name])
     np.testing.assert_array_equal(s.pair_force(), arr[f"{name}/fpair"])
     np.testing.assert_array_equal(s.chain_force(), arr[f"{name}/fchain"])
 
 
-@pytest.mark.parametrize("name", ["L500", "B400"])
+@pytest.mark.parametrize("name", STEP0)
 def test_energies_and_dpotential_match_reference(golden_step0, golden_traj, name):
     meta, _ = golden_step0
     m = meta[name]
-    s, _ = _system(golden_traj, "B400t" if name == "B400" else name)
+    s, _ = _system(golden_traj, SRC[name])
     scale = meta["scale"]
     assert s.pair_potential() == m["U_pair"]
     assert s.chain_potential() == pytest.approx(m["U_chain"], rel=1e-9, abs=1e-24)
@@ -272,12 +277,12 @@ def test_langevin_noise_moments():
     assert s.acc.var() == pytest.approx(sigma2 / 3, rel=0.02)
 
 
-@pytest.mark.parametrize("name", ["L500", "B400t"])
+@pytest.mark.parametrize("name", ["L500", "B400t", "A500", "AB400t"])
 def test_md_loop_reproduces_reference_binary(golden_traj, name):
     """The compiled MD binary (one thread) vs the oracle loop with the reference's MT19937
     streams: same state after N steps, same measured series, same MC decisions."""
     s, meta = _system(golden_traj, name)
-    steps = {"L500": 20, "B400t": 40}[name]
+    steps = {"L500": 20, "B400t": 40, "A500": 20, "AB400t": 40}[name]
     md = O.MD(s, rng_mode=0, mt_streams=1)
     md.init_forces()
     U0, K0 = md.measure()
@@ -296,7 +301,61 @@ def test_md_loop_reproduces_reference_binary(golden_traj, name):
     np.testing.assert_allclose(s.box, final["size"], rtol=1e-14)
     for (i, U, K), (tp, up), (tk, kk) in zip(measured, pot[1:], kin[1:]):
         assert float(f"{U:.6g}") == up and float(f"{K:.6g}") == kk
-    if name == "B400t":
+    if name + "/resizeHist" in golden_traj:
         hist = golden_traj[name + "/resizeHist"]
         accepted = int(round(hist[:, 1].sum()))
-        assert md.trials() == 4 and accepted == 2
+        assert md.trials() == 4 and bin(md.decisions() & 0xF).count("1") == accepted
+        if name == "B400t":
+            assert accepted == 2
+
+
+def _owner_rows(s):
+    """Per pair within the cutoff (i < j, minimum image): the row index of the particle the
+    reference's half-shell traversal makes first, restated without cells -- across cells
+    the one whose forward half shell (cellOpt.h:715) holds the other's cell, within a cell
+    the lower index (its list runs in descending index, cellOpt.h:572-585)."""
+    nc, cs = s.cell_grid()
+    c = np.minimum((s.pos / cs).astype(np.int64), nc - 1)
+    nb = [(1, 0, 0), (0, 1, 0), (1, 1, 0), (0, 0, 1), (1, 0, 1), (0, 1, 1), (1, 1, 1), (-1, 0, 1), (0, -1, 1),
+          (-1, 1, 1), (1, -1, 1), (-1, -1, 1), (1, -1, 0)]
+    fwd = {tuple(x) for x in nb}
+    d = s.pos[:, None, :] - s.pos[None, :, :]
+    d -= s.box * np.round(d / s.box)
+    r2 = (d ** 2).sum(-1)
+    i, j = np.nonzero(np.triu(r2 < CUT * CUT, 1))
+    off = (c[j] - c[i] + 1) % nc - 1          # j's cell seen from i's, each axis in {-1, 0, 1}
+    same = np.all(off == 0, axis=1)
+    i_first = same | np.array([tuple(o) in fwd for o in off.tolist()])
+    return i, j, i_first, d[i, j]
+
+
+@pytest.mark.parametrize("name", ["A500", "AB400"])
+def test_asymmetric_table_rows_follow_traversal_order(golden_step0, golden_traj, name):
+    """With (2, 3) and (3, 2) rows that differ, the reference's forces are those of each
+    pair's first particle's row: an O(N^2) evaluation of that rule matches the compiled
+    reference's step-0 forces, and the other rule (the second particle's row) does not."""
+    _, arr = golden_step0
+    s, _ = _system(golden_traj, SRC[name])
+    nT = int(round(np.sqrt(len(s.fconst) / 6)))
+    F = s.fconst.reshape(nT * nT, 6)
+    i, j, i_first, dij = _owner_rows(s)
+    assert i_first.any() and (~i_first).any()
+    r = np.sqrt((dij ** 2).sum(1))
+
+    def forces(first_is_i):
+        row = np.where(first_is_i, s.type[i] * nT + s.type[j], s.type[j] * nT + s.type[i])
+        C = F[row]
+        o = np.where(r >= C[:, 0], 3, 0)
+        c0, c1, c2 = (np.take_along_axis(C, (o + k)[:, None], 1)[:, 0] for k in range(3))
+        m = c0 - r
+        mag = (c1 - c2 * m) * m / r
+        f = np.zeros_like(s.pos)
+        np.add.at(f, i, dij * mag[:, None])
+        np.add.at(f, j, -dij * mag[:, None])
+        return f
+
+    want = arr[f"{name}/fpair"]
+    scale = np.abs(want).max()
+    np.testing.assert_allclose(forces(i_first), want, rtol=0, atol=1e-10 * scale)
+    assert np.abs(forces(~i_first) - want).max() > 1e-3 * scale
+
