@@ -163,7 +163,7 @@ def bench_batched(steps: int, warmup: int, seq_steps: int) -> dict:
     b = sm.Solver.batch(systems)
     b.init_forces()
     tuned_ms = b.autotune(32)
-    knobs = {k: b.tune_get(k) for k in ("pair_chunk", "pair_variant", "graph")}
+    knobs = {k: b.tune_get(k) for k in ("pair_chunk", "pair_variant", "graph", "sort_mode")}
     b.run(1, warmup)
     b.sync()
     torch.cuda.synchronize()
@@ -179,9 +179,9 @@ def bench_batched(steps: int, warmup: int, seq_steps: int) -> dict:
         f"{batched:.4g} particle-steps/s, knobs {knobs}, MC acceptance {accepted.sum() / max(trials.sum(), 1):.3f}")
 
     # the plain solver on each replica alone, at that replica's own best launch shape: the
-    # autotuner picks its pair chunk and the faster of captured graph and eager stepping
-    # (where a step without a box move ends in the fused kick + next-first-half kernel);
-    # the same steps and warmup as the batch, summed walls
+    # autotuner picks its pair chunk, sort mode and the faster of captured graph and eager
+    # stepping (where a step without a box move ends in the fused kick + next-first-half
+    # kernel); the same steps and warmup as the batch, summed walls
     seq_wall, seq_work, modes = 0.0, 0, {"graph": 0, "eager": 0}
     for d in systems:
         s = sm.Solver.from_system(d)

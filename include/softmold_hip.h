@@ -175,12 +175,17 @@ int sm_potential_delta(sm_ctx *ctx, const double *scale, double *du_pair, double
  * the pair-force kernel (1..4096); "pair_variant" = 0 one-phase force kernel with the
  * home cell's candidates culled per quarter cell (default), 1 two-phase (FP32 screen into
  * per-lane hit lists, then the FP64 test on the hits);
- * "graph" = 1 to replay sm_run's aligned 8-step blocks as one captured hipGraph. */
+ * "graph" = 1 to replay sm_run's aligned 8-step blocks as one captured hipGraph;
+ * "sort_mode" = 1 to re-sort the cell keys incrementally -- the beads whose cell changed
+ * are sorted and merged into the previous order (the default from a million beads up) --
+ * or 0 to sort every key each build.  Both give the same order, so neither changes a
+ * result. */
 int sm_tune(sm_ctx *ctx, const char *knob, int32_t value);
 int sm_tune_get(sm_ctx *ctx, const char *knob, int32_t *value);
 /* Autotuner (no reference counterpart): times `steps` steps of this context's own work
  * for each pair_chunk in {2,4,8,16,32,64} with the one-phase kernel and the graph on, then
- * the best without the graph, from one saved state; keeps the fastest setting, restores
+ * the best with the other sort_mode, then the best without the graph, from one saved
+ * state; keeps the fastest setting, restores
  * the state, and returns the best ms per step.  These settings leave every result
  * bit-identical, so tuning never changes a trajectory. */
 int sm_autotune(sm_ctx *ctx, int32_t steps, double *best_ms_per_step);

@@ -73,7 +73,7 @@ struct DevGlobal {
     long long step;            // step index of the step in progress (Philox counter)
 };
 
-enum : int { FLAG_BAD_POSITION = 1, FLAG_CELL_OVERFLOW = 2, FLAG_SORT_TIMEOUT = 4 };
+enum : int { FLAG_BAD_POSITION = 1, FLAG_CELL_OVERFLOW = 2, FLAG_SORT_TIMEOUT = 4, FLAG_SORT_COUNT = 8 };
 
 struct Params {
     int n;          // beads, all replicas
@@ -107,9 +107,17 @@ void launch_seg_sum(hipStream_t st, const SegTiles &S, const double *x, double *
 // step_set >= 0: the device step counter is set to step_set; -1: it is advanced by one
 // (the form a captured graph replays); <= -2: left as it is.  If ghist != NULL the digit
 // histograms of the first `passes` 8-bit digits of the keys are added to it.
+// The incremental re-sort's bookkeeping in the key kernels (cnt_b == NULL: none; the
+// histograms then count every key): a bead moves when its key differs from kold (every
+// bead when kold is NULL); the movers' keys go into the histograms and each block of 256
+// beads writes its mover count to cnt_b[block].
+struct MoveBook {
+    const uint32_t *kold;
+    uint32_t *cnt_b;
+};
 void launch_first(hipStream_t st, const Params &P, Bead *beads, float4 *vel, const float4 *acc, const DevRep *reps,
                   DevGlobal *glob, uint32_t *key, uint32_t *idx, int integrate, long long step_set, uint32_t *ghist,
-                  int passes);
+                  int passes, const MoveBook &book);
 // also zeroes clear[0 .. clear_words) (the next sort's scratch)
 void launch_reorder(hipStream_t st, int n, const uint32_t *skey, const uint32_t *sidx, const Bead *beads,
                     Bead *sbeads, int *cstart, int *cend, uint32_t *clear, size_t clear_words);
@@ -134,7 +142,7 @@ enum : int { KICK_STEP = 0, KICK_INIT = 1, KICK_NONE = 2 };
 void launch_kick_first(hipStream_t st, const Params &P, const Bead *beads, Bead *beads_out, const int2 *chain,
                        const double4 *cconst, const GenTables &gen, const float4 *fpair, float4 *vel, float4 *acc,
                        const DevRep *reps, DevGlobal *glob, long long step, uint32_t *key, uint32_t *idx,
-                       uint32_t *ghist, int passes);
+                       uint32_t *ghist, int passes, const MoveBook &book);
 void launch_second(hipStream_t st, int n, const Bead *beads, const float4 *acc, float4 *vel, float dt);
 
 // FP64 gathers over the 27-cell stencil.  GATHER_U / GATHER_DU write one FP64 value per
@@ -189,16 +197,38 @@ void launch_head_radius(hipStream_t st, int nrep, const double *sums, const doub
 struct SortBuffers {
     uint32_t *k0, *v0, *k1, *v1;
     uint32_t *scratch[2];   // sort_scratch_words(n, 4) words each, double-buffered
-    int *errflag;           // receives FLAG_SORT_TIMEOUT
+    int *errflag;           // receives FLAG_SORT_TIMEOUT / FLAG_SORT_COUNT
     int n;
     int tiles;
 };
 size_t sort_scratch_words(int n, int max_passes);
 size_t sort_hist_words();   // the digit histograms at the start of the scratch
+// the incremental re-sort's tiles: kMoveTile beads (or previous slots) each; the key
+// kernel's blocks of 256 beads write one mover count each (move_blocks of them)
+constexpr int kMoveTile = 4096;
+int move_tiles(int n);
+int move_blocks(int n);
 int sort_passes(int bits);
 // Stable LSD sort of (k0,v0) by the low `bits` bits using scratch[parity], whose digit
 // histograms k_first has filled and whose other words are zero; returns which buffer pair
 // holds the result (0 -> k0/v0, 1 -> k1/v1).
 int radix_sort(hipStream_t st, SortBuffers &b, int bits, int parity);
+// Incremental re-sort: the new sorted order from the previous one and the beads whose key
+// changed (whose digit histograms and per-tile counts the key kernel has filled into
+// scratch[parity]); (k0,v0) and (k1,v1) hold the movers.  The result is the one radix_sort
+// gives for the same keys.
+struct MoveSort {
+    const uint32_t *knew;                    // keys by bead index, this build
+    const uint32_t *kold;                    // keys by bead index at the previous sort, or NULL: all move
+    const uint32_t *skey_old, *sidx_old;     // the previous sorted order (unused when kold is NULL)
+    const int *cstart, *cend;                // the previous dense cell table
+    const uint32_t *cnt_b;                   // movers per key-kernel block (move_blocks)
+    uint32_t *cnt_i, *cnt_s;                 // movers per index tile and per previous-slot tile (move_tiles)
+    unsigned long long *mflags;              // per 64 previous slots: which beads moved
+    uint32_t *mbefore;                       // per previous slot: movers before it, bit 31 = it moves
+    int *mcount;                             // the mover count
+    uint32_t *skey, *sidx;                   // the new sorted order
+};
+void sort_moved(hipStream_t st, SortBuffers &b, const MoveSort &m, int bits, int parity);
 
 }  // namespace smdev

@@ -106,11 +106,26 @@ __device__ inline uint32_t first_one(int i, Bead *__restrict__ beads, float4 *__
     return bead_key(b, R, glob);
 }
 
+// The incremental re-sort's mover test for bead i with key k (see MoveBook); returns
+// whether the key goes into the histograms.
+__device__ inline bool note_move(const MoveBook &book, int i, uint32_t k, bool valid)
+{
+    if (!valid || !book.cnt_b) return valid;
+    return book.kold == nullptr || book.kold[i] != k;
+}
+
+// the block's mover count (every block writes one)
+__device__ inline void flush_moves(const MoveBook &book, int nmoved)
+{
+    if (book.cnt_b && threadIdx.x == 0) book.cnt_b[blockIdx.x] = (uint32_t)nmoved;
+}
+
 __global__ __launch_bounds__(kBlock) void k_first(int n, Bead *__restrict__ beads, float4 *__restrict__ vel,
                                                   const float4 *__restrict__ acc, const DevRep *__restrict__ reps,
                                                   DevGlobal *__restrict__ glob, uint32_t *__restrict__ key,
                                                   uint32_t *__restrict__ idx, float halfdt, double dt, int integrate,
-                                                  long long step_set, uint32_t *__restrict__ ghist, int passes)
+                                                  long long step_set, uint32_t *__restrict__ ghist, int passes,
+                                                  MoveBook book)
 {
     __shared__ uint32_t h[4][256];
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -123,13 +138,15 @@ __global__ __launch_bounds__(kBlock) void k_first(int n, Bead *__restrict__ bead
     if (i < n) k = first_one(i, beads, vel, acc, reps, glob, halfdt, dt, integrate);
     if (ghist) {
         // the sort's digit histograms while the key is in a register (all lanes: ballots)
-        hist_add(h, k, i < n, passes);
-        __syncthreads();
+        const bool counted = note_move(book, i, k, i < n);
+        hist_add(h, k, counted, passes);
+        const int nmoved = __syncthreads_count(counted);
         hist_flush(h, ghist, passes);
+        flush_moves(book, nmoved);
     }
     if (i < n) {
         key[i] = k;
-        idx[i] = (uint32_t)i;
+        if (idx) idx[i] = (uint32_t)i;
     }
 }
 
@@ -334,7 +351,7 @@ __global__ __launch_bounds__(kBlock) void k_kick_first(int n, const Bead *__rest
                                                        DevGlobal *__restrict__ glob, float halfdt, double dt,
                                                        long long step, uint32_t *__restrict__ key,
                                                        uint32_t *__restrict__ idx, uint32_t *__restrict__ ghist,
-                                                       int passes)
+                                                       int passes, MoveBook book)
 {
     __shared__ uint32_t h[4][256];
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -358,12 +375,14 @@ __global__ __launch_bounds__(kBlock) void k_kick_first(int n, const Bead *__rest
         beads_out[i] = b;
         k = bead_key(b, R, glob);
     }
-    hist_add(h, k, i < n, passes);
-    __syncthreads();
+    const bool counted = note_move(book, i, k, i < n);
+    hist_add(h, k, counted, passes);
+    const int nmoved = __syncthreads_count(counted);
     hist_flush(h, ghist, passes);
+    flush_moves(book, nmoved);
     if (i < n) {
         key[i] = k;
-        idx[i] = (uint32_t)i;
+        if (idx) idx[i] = (uint32_t)i;
     }
 }
 
@@ -738,10 +757,10 @@ inline int blocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 // ---------------------------------------------------------------------------------------
 void launch_first(hipStream_t st, const Params &P, Bead *beads, float4 *vel, const float4 *acc, const DevRep *reps,
                   DevGlobal *glob, uint32_t *key, uint32_t *idx, int integrate, long long step_set, uint32_t *ghist,
-                  int passes)
+                  int passes, const MoveBook &book)
 {
     hipLaunchKernelGGL(k_first, dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, beads, vel, acc, reps, glob, key, idx,
-                       (float)(0.5 * P.dt), P.dt, integrate, step_set, ghist, passes);
+                       (float)(0.5 * P.dt), P.dt, integrate, step_set, ghist, passes, book);
 }
 
 void launch_reorder(hipStream_t st, int n, const uint32_t *skey, const uint32_t *sidx, const Bead *beads,
@@ -762,10 +781,10 @@ void launch_bonded_kick(hipStream_t st, const Params &P, const Bead *beads, cons
 void launch_kick_first(hipStream_t st, const Params &P, const Bead *beads, Bead *beads_out, const int2 *chain,
                        const double4 *cconst, const GenTables &gen, const float4 *fpair, float4 *vel, float4 *acc,
                        const DevRep *reps, DevGlobal *glob, long long step, uint32_t *key, uint32_t *idx,
-                       uint32_t *ghist, int passes)
+                       uint32_t *ghist, int passes, const MoveBook &book)
 {
     hipLaunchKernelGGL(k_kick_first, dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, beads, beads_out, chain, cconst, gen,
-                       fpair, vel, acc, reps, glob, (float)(P.dt * 0.5), P.dt, step, key, idx, ghist, passes);
+                       fpair, vel, acc, reps, glob, (float)(P.dt * 0.5), P.dt, step, key, idx, ghist, passes, book);
 }
 
 void launch_second(hipStream_t st, int n, const Bead *beads, const float4 *acc, float4 *vel, float halfdt)
