@@ -12,8 +12,10 @@
 // written out from f(r) and f'(r); for CHAIN bonds and bends it is the exact directional
 // derivative of the force code evaluated on dual numbers.
 //
-// These kernels serve short fitting trajectories of small systems: one thread per bead,
-// stencil cells read straight from global memory, no LDS staging.
+// These kernels serve short fitting trajectories of small systems: stencil cells are read
+// straight from global memory, no LDS staging.  The pair term gives each bead 32 lanes,
+// one per stencil cell, and combines them by a fixed shuffle tree; with one thread per
+// bead the 12k-bead bilayer filled 47 workgroups and the pair term took 465 us a call.
 //
 // Reference: force law Force<double> (MD.h:795-848), potential (MD.h:895-930), constant
 // tables laradjiRevaleeFC/PC (potentials/laradjiRevalee.h:84-152), harmonicF/bendF
@@ -142,6 +144,16 @@ __host__ __device__ constexpr uint32_t forward_cells()
     return m;
 }
 
+// lanes of a bead's stencil in k_adj_pair: one per stencil cell (27 used)
+constexpr int kAdjLanes = 32;
+
+__device__ inline double lane_sum(double x)
+{
+#pragma unroll
+    for (int o = kAdjLanes / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kAdjLanes);
+    return x;
+}
+
 __global__ __launch_bounds__(kBlock) void k_adj_pair(int n, int nT, double cut2, const Bead *__restrict__ sb,
                                                      const uint32_t *__restrict__ skey,
                                                      const uint32_t *__restrict__ sidx,
@@ -152,87 +164,102 @@ __global__ __launch_bounds__(kBlock) void k_adj_pair(int n, int nT, double cut2,
                                                      double4 *__restrict__ hvout, double *__restrict__ g, double obj,
                                                      int want_hv)
 {
-    const int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n) return;
-    const Bead hb = sb[s];
-    const uint32_t me = sidx[s];
-    const DevRep &R = reps[hb.rep];
-    const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
-    const uint32_t kbase = (uint32_t)R.cellbase;
-    const uint32_t k = skey[s] - kbase;
-    const int kx = (int)(k % (uint32_t)nx), ky = (int)((k / (uint32_t)nx) % (uint32_t)ny),
-              kz = (int)(k / (uint32_t)(nx * ny));
-    const double4 wi = w[me];
-    const size_t tsz = 6 * (size_t)nT * nT;
+    const long long gt = (long long)blockIdx.x * kBlock + threadIdx.x;
+    const int s = (int)(gt / kAdjLanes);   // sorted slot of the bead (uniform over its lanes)
+    const int c = (int)(gt % kAdjLanes);   // its stencil cell
+    const bool live = s < n;
     double Fx = 0, Fy = 0, Fz = 0, Hx = 0, Hy = 0, Hz = 0;
     double gw[2] = {0.0, 0.0}, du[2] = {0.0, 0.0};
-    for (int c = 0; c < 27; c++) {
-        int X0 = kx + c % 3 - 1, Y0 = ky + (c / 3) % 3 - 1, Z0 = kz + c / 9 - 1;
-        const double sx = X0 < 0 ? -R.L[0] : (X0 >= nx ? R.L[0] : 0.0);
-        const double sy = Y0 < 0 ? -R.L[1] : (Y0 >= ny ? R.L[1] : 0.0);
-        const double sz = Z0 < 0 ? -R.L[2] : (Z0 >= nz ? R.L[2] : 0.0);
-        X0 = X0 < 0 ? X0 + nx : (X0 >= nx ? X0 - nx : X0);
-        Y0 = Y0 < 0 ? Y0 + ny : (Y0 >= ny ? Y0 - ny : Y0);
-        Z0 = Z0 < 0 ? Z0 + nz : (Z0 >= nz ? Z0 - nz : Z0);
-        const uint32_t kk = kbase + (uint32_t)(X0 + Y0 * nx + Z0 * nx * ny);
-        const int c0 = cstart[kk];
-        if (!((unsigned)c0 < (unsigned)n && skey[c0] == kk)) continue;
-        const int c1 = cend[kk];
-        for (int j = c0; j < c1; j++) {
-            if (j == s) continue;
-            const Bead q = sb[j];
-            const double dx = hb.x - (q.x + sx), dy = hb.y - (q.y + sy), dz = hb.z - (q.z + sz);
-            const double r2 = sq3(dx, dy, dz);
-            if (!(r2 < cut2)) continue;
-            const double r = sqrt(r2);
-            // the row of the particle the reference's traversal makes first: across cells
-            // the one whose forward half shell (cellOpt.h:715) holds the other's cell,
-            // within a cell the lower index (cellOpt.h:572-585,780-795)
-            constexpr uint32_t fwd = forward_cells();
-            const bool first = c == 13 ? me < sidx[j] : ((fwd >> c) & 1u) != 0;
-            const size_t row = 6 * (first ? (size_t)hb.type * nT + q.type : (size_t)q.type * nT + hb.type);
-            const double *C = fc + row;
-            const int o = (r >= C[0]) ? 3 : 0;
-            const double m = C[o] - r;
-            const double f = (C[o + 1] - C[o + 2] * m) * m;        // force along d/r
-            const double fr = f / r;
-            Fx += dx * fr;
-            Fy += dy * fr;
-            Fz += dz * fr;
-            if (want_hv) {
-                const double4 wj = w[sidx[j]];
-                const double ux = wi.x - wj.x, uy = wi.y - wj.y, uz = wi.z - wj.z;
-                const double fp = -(C[o + 1] - 2.0 * C[o + 2] * m);  // df/dr
-                const double e = (dx * ux + dy * uy + dz * uz) / r;  // dr along the direction
-                const double t = (fp / r - f / r2) * e;
-                Hx += ux * fr + dx * t;
-                Hy += uy * fr + dy * t;
-                Hz += uz * fr + dz * t;
-            }
-            const double wd = (wi.x * dx + wi.y * dy + wi.z * dz) / r;
-            const double *U = duc + row;   // only C1, C2, C4, C5 of the U row depend on theta
-            for (int p = 0; p < 2; p++) {
-                const double *D = dfc + p * tsz + row;
-                gw[p] += wd * ((D[o + 1] - D[o + 2] * m) * m);
-                if (obj != 0.0) {
-                    const double *E = U + p * tsz;
-                    double d;
-                    if (r <= C[0]) {
-                        const double mu = C[0] - r;
-                        d = E[1] * mu * mu + E[2];
-                    } else {
-                        const double mu = C[3] - r;
-                        d = mu * mu * (E[4] - mu * E[5]);
+    uint32_t me = 0;
+    if (live) {
+        const Bead hb = sb[s];
+        me = sidx[s];
+        const DevRep &R = reps[hb.rep];
+        const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
+        const uint32_t kbase = (uint32_t)R.cellbase;
+        const uint32_t k = skey[s] - kbase;
+        const int kx = (int)(k % (uint32_t)nx), ky = (int)((k / (uint32_t)nx) % (uint32_t)ny),
+                  kz = (int)(k / (uint32_t)(nx * ny));
+        const double4 wi = w[me];
+        const size_t tsz = 6 * (size_t)nT * nT;
+        if (c < 27) {
+            int X0 = kx + c % 3 - 1, Y0 = ky + (c / 3) % 3 - 1, Z0 = kz + c / 9 - 1;
+            const double sx = X0 < 0 ? -R.L[0] : (X0 >= nx ? R.L[0] : 0.0);
+            const double sy = Y0 < 0 ? -R.L[1] : (Y0 >= ny ? R.L[1] : 0.0);
+            const double sz = Z0 < 0 ? -R.L[2] : (Z0 >= nz ? R.L[2] : 0.0);
+            X0 = X0 < 0 ? X0 + nx : (X0 >= nx ? X0 - nx : X0);
+            Y0 = Y0 < 0 ? Y0 + ny : (Y0 >= ny ? Y0 - ny : Y0);
+            Z0 = Z0 < 0 ? Z0 + nz : (Z0 >= nz ? Z0 - nz : Z0);
+            const uint32_t kk = kbase + (uint32_t)(X0 + Y0 * nx + Z0 * nx * ny);
+            const int c0 = cstart[kk];
+            const int c1 = ((unsigned)c0 < (unsigned)n && skey[c0] == kk) ? cend[kk] : c0;
+            for (int j = c0; j < c1; j++) {
+                if (j == s) continue;
+                const Bead q = sb[j];
+                const double dx = hb.x - (q.x + sx), dy = hb.y - (q.y + sy), dz = hb.z - (q.z + sz);
+                const double r2 = sq3(dx, dy, dz);
+                if (!(r2 < cut2)) continue;
+                const double r = sqrt(r2);
+                // the row of the particle the reference's traversal makes first: across
+                // cells the one whose forward half shell (cellOpt.h:715) holds the other's
+                // cell, within a cell the lower index (cellOpt.h:572-585,780-795)
+                constexpr uint32_t fwd = forward_cells();
+                const bool first = c == 13 ? me < sidx[j] : ((fwd >> c) & 1u) != 0;
+                const size_t row = 6 * (first ? (size_t)hb.type * nT + q.type : (size_t)q.type * nT + hb.type);
+                const double *C = fc + row;
+                const int o = (r >= C[0]) ? 3 : 0;
+                const double m = C[o] - r;
+                const double f = (C[o + 1] - C[o + 2] * m) * m;        // force along d/r
+                const double fr = f / r;
+                Fx += dx * fr;
+                Fy += dy * fr;
+                Fz += dz * fr;
+                if (want_hv) {
+                    const double4 wj = w[sidx[j]];
+                    const double ux = wi.x - wj.x, uy = wi.y - wj.y, uz = wi.z - wj.z;
+                    const double fp = -(C[o + 1] - 2.0 * C[o + 2] * m);  // df/dr
+                    const double e = (dx * ux + dy * uy + dz * uz) / r;  // dr along the direction
+                    const double t = (fp / r - f / r2) * e;
+                    Hx += ux * fr + dx * t;
+                    Hy += uy * fr + dy * t;
+                    Hz += uz * fr + dz * t;
+                }
+                const double wd = (wi.x * dx + wi.y * dy + wi.z * dz) / r;
+                const double *U = duc + row;   // only C1, C2, C4, C5 of the U row depend on theta
+                for (int p = 0; p < 2; p++) {
+                    const double *D = dfc + p * tsz + row;
+                    gw[p] += wd * ((D[o + 1] - D[o + 2] * m) * m);
+                    if (obj != 0.0) {
+                        const double *E = U + p * tsz;
+                        double d;
+                        if (r <= C[0]) {
+                            const double mu = C[0] - r;
+                            d = E[1] * mu * mu + E[2];
+                        } else {
+                            const double mu = C[3] - r;
+                            d = mu * mu * (E[4] - mu * E[5]);
+                        }
+                        du[p] += 0.5 * d;
                     }
-                    du[p] += 0.5 * d;
                 }
             }
         }
     }
-    fout[me] = make_double4(Fx, Fy, Fz, 0.0);
-    if (want_hv) hvout[me] = make_double4(Hx, Hy, Hz, 0.0);
-    g[me] += gw[0] + obj * du[0];
-    g[(size_t)n + me] += gw[1] + obj * du[1];
+    // every lane of the group takes part in the tree, live or not
+    Fx = lane_sum(Fx);
+    Fy = lane_sum(Fy);
+    Fz = lane_sum(Fz);
+    Hx = lane_sum(Hx);
+    Hy = lane_sum(Hy);
+    Hz = lane_sum(Hz);
+    const double g0 = lane_sum(gw[0] + obj * du[0]);
+    const double g1 = lane_sum(gw[1] + obj * du[1]);
+    if (live && c == 0) {
+        fout[me] = make_double4(Fx, Fy, Fz, 0.0);
+        if (want_hv) hvout[me] = make_double4(Hx, Hy, Hz, 0.0);
+        g[me] += g0;
+        g[(size_t)n + me] += g1;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_adj_chain(int n, const Bead *__restrict__ beads,
@@ -349,7 +376,8 @@ void launch_adj_pair(hipStream_t st, const Params &P, const Bead *sbeads, const 
                      const int *cstart, const int *cend, const DevRep *reps, const AdjTables &T, const double4 *w,
                      double4 *fout, double4 *hvout, double *g, double obj, int want_hv)
 {
-    hipLaunchKernelGGL(k_adj_pair, dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, P.nT, P.cutoff * P.cutoff, sbeads,
+    hipLaunchKernelGGL(k_adj_pair, dim3(blocks((long long)P.n * kAdjLanes)), dim3(kBlock), 0, st, P.n, P.nT,
+                       P.cutoff * P.cutoff, sbeads,
                        skey, sidx, cstart, cend, reps, T.fc, T.dfc, T.duc, w, fout, hvout, g, obj, want_hv);
 }
 
