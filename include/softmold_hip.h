@@ -209,10 +209,11 @@ int sm_param_derivative(sm_ctx *ctx, int32_t type_a, int32_t type_b, double *out
 int sm_get_params(sm_ctx *ctx, int32_t type_a, int32_t type_b, double *theta);
 /* From the context's current positions and velocities: initial forces, then steps
  * 1..nsteps with the counter RNG (box moves as configured), J = mean over the steps of
- * U_pair + U_chain after each step's kick.  If grad != NULL it receives dJ/dtheta (4
- * values) from the discrete adjoint of the step in FP64, recomputing the trajectory from
- * checkpoints every `checkpoint` steps; MC accept/reject decisions are held fixed (the
- * pathwise derivative).  The context state is left as it was.  Single-system contexts. */
+ * U_pair + U_chain (with the generic records' energy) after each step's kick, one value
+ * per replica.  If grad != NULL it receives dJ/dtheta (4 values per replica, replica
+ * after replica) from the discrete adjoint of the step in FP64, recomputing the trajectory
+ * from checkpoints every `checkpoint` steps; MC accept/reject decisions are held fixed
+ * (the pathwise derivative).  The context state is left as it was. */
 int sm_adjoint(sm_ctx *ctx, int32_t type_a, int32_t type_b, int32_t nsteps, int32_t checkpoint, double *J,
                double *grad);
 

@@ -52,12 +52,16 @@ def get_params(solver: Solver, pair: tuple[int, int]) -> np.ndarray:
 
 
 def objective(solver: Solver, pair: tuple[int, int], nsteps: int, checkpoint: int, grad: bool = True):
-    """(J, dJ/dtheta or None) at the solver's current constants; the solver state is kept."""
-    J = C.c_double()
-    g = np.zeros(4)
-    _check(lib().sm_adjoint(solver.h, int(pair[0]), int(pair[1]), int(nsteps), int(checkpoint), C.byref(J),
+    """(J, dJ/dtheta or None) at the solver's current constants; the solver state is kept.
+    A batch gives one J per replica and a [replicas, 4] gradient."""
+    R = solver.nrep
+    J = np.zeros(R)
+    g = np.zeros((R, 4))
+    _check(lib().sm_adjoint(solver.h, int(pair[0]), int(pair[1]), int(nsteps), int(checkpoint), _dp(J),
                             _dp(g) if grad else None))
-    return J.value, (g if grad else None)
+    if R == 1:
+        return float(J[0]), (g[0] if grad else None)
+    return J, (g if grad else None)
 
 
 try:

@@ -329,6 +329,54 @@ __global__ __launch_bounds__(kBlock) void k_adj_chain(int n, const Bead *__restr
     g[3 * (size_t)n + i] += gt;
 }
 
+// Generic BOND / BEND / BALL records (doBondForce, doBendForce, doBallForce,
+// system.h:1881-2035): per bead, the force of each record it belongs to, from its side, as
+// k_bonded_kick takes it, evaluated on dual numbers along w for (dF/dx) w.  Their constants
+// are not among theta, so they add nothing to g.
+__global__ __launch_bounds__(kBlock) void k_adj_gen(int n, GenTables gen, const Bead *__restrict__ beads,
+                                                    const DevRep *__restrict__ reps, const double4 *__restrict__ w,
+                                                    double4 *__restrict__ fout, double4 *__restrict__ hvout,
+                                                    int want_hv)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int e0 = gen.gptr[i], e1 = gen.gptr[i + 1];
+    if (e0 == e1) return;
+    const DevRep &R = reps[beads[i].rep];
+    const double L[3] = {R.L[0], R.L[1], R.L[2]};
+    auto at = [&](int q) {
+        const Bead b = beads[q];
+        const double4 wq = want_hv ? w[q] : make_double4(0, 0, 0, 0);
+        return V3<Dual>{Dual{b.x, wq.x}, Dual{b.y, wq.y}, Dual{b.z, wq.z}};
+    };
+    const Dual z{0.0, 0.0};
+    V3<Dual> f{z, z, z};
+    for (int e = e0; e < e1; e++) {
+        const GenEnt g = gen.gent[e];
+        const double2 K = gen.gconst[e];
+        V3<Dual> fr;
+        if (g.kind == 0 || g.kind == 2) {
+            const V3<Dual> d = sub_mi(at(g.p1), at(g.p2), L);
+            // BALL (kind 2, harmonicHalfF, MD.h:1164-1187): only inside r0
+            const bool on = g.kind == 0 || d.x.v * d.x.v + d.y.v * d.y.v + d.z.v * d.z.v < K.x * K.x;
+            fr = on ? bond_f(d, K.x, K.y) : V3<Dual>{z, z, z};
+            if (i != g.p1) fr = V3<Dual>{-fr.x, -fr.y, -fr.z};
+        } else {
+            const V3<Dual> q1 = at(g.p1), q2 = at(g.p2), q3 = at(g.p3);
+            fr = bend_f(sub_mi(q1, q2, L), sub_mi(q2, q3, L), K.x, K.y, i == g.p1 ? 0 : (i == g.p2 ? 1 : 2));
+        }
+        f = V3<Dual>{f.x + fr.x, f.y + fr.y, f.z + fr.z};
+    }
+    double4 fo = fout[i];
+    fo.x += f.x.v; fo.y += f.y.v; fo.z += f.z.v;
+    fout[i] = fo;
+    if (want_hv) {
+        double4 h = hvout[i];
+        h.x += f.x.d; h.y += f.y.d; h.z += f.z.d;
+        hvout[i] = h;
+    }
+}
+
 // Reverse of one step s, given the adjoints (X, V, A) of (x_s, v_s, a_s), where A already
 // holds c V_s for type != 0 (the v_s = v_h + c a_s term):
 //   X~ = S X (accepted MC move) - obj F(x~_s) + (dF/dx) A     (objective obj U(x~_s))
@@ -379,6 +427,13 @@ void launch_adj_pair(hipStream_t st, const Params &P, const Bead *sbeads, const 
     hipLaunchKernelGGL(k_adj_pair, dim3(blocks((long long)P.n * kAdjLanes)), dim3(kBlock), 0, st, P.n, P.nT,
                        P.cutoff * P.cutoff, sbeads,
                        skey, sidx, cstart, cend, reps, T.fc, T.dfc, T.duc, w, fout, hvout, g, obj, want_hv);
+}
+
+void launch_adj_gen(hipStream_t st, const Params &P, const GenTables &gen, const Bead *beads, const DevRep *reps,
+                    const double4 *w, double4 *fout, double4 *hvout, int want_hv)
+{
+    hipLaunchKernelGGL(k_adj_gen, dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, gen, beads, reps, w, fout, hvout,
+                       want_hv);
 }
 
 void launch_adj_chain(hipStream_t st, const Params &P, const Bead *beads, const int2 *chain, const double4 *cconst,

@@ -24,6 +24,9 @@ void launch_adj_pair(hipStream_t st, const Params &P, const Bead *sbeads, const 
 void launch_adj_chain(hipStream_t st, const Params &P, const Bead *beads, const int2 *chain, const double4 *cconst,
                       const DevRep *reps, const double4 *w, double4 *fout, double4 *hvout, double *g, double obj,
                       int want_hv);
+// Per bead: the generic BOND / BEND / BALL records' force (+=) and its (dF/dx) w (+=).
+void launch_adj_gen(hipStream_t st, const Params &P, const GenTables &gen, const Bead *beads, const DevRep *reps,
+                    const double4 *w, double4 *fout, double4 *hvout, int want_hv);
 // One reverse step (see sm_adjoint.hip); post = the replicas after this step's MC move.
 void launch_adj_step(hipStream_t st, const Params &P, const Bead *beads, const DevRep *post, int mc, double4 *X,
                      double4 *V, double4 *A, const double4 *F, const double4 *HV, double obj, int prep_next);

@@ -1769,9 +1769,8 @@ int sm_adjoint(sm_ctx *c, int32_t type_a, int32_t type_b, int32_t nsteps, int32_
 {
     return guarded([&] {
         if (!c || !J) invalid("null argument");
-        if (c->nrep != 1) unsupported("the adjoint runs on single-system contexts");
-        if (c->gptr) unsupported("the adjoint covers CHAIN molecules only (no BOND / BEND records)");
         if (nsteps < 1 || checkpoint < 1 || checkpoint > nsteps) invalid("need 1 <= checkpoint <= nsteps");
+        const int R = c->nrep;
         check_pair(c, type_a, type_b);
         const int n = c->n, S = nsteps, k = checkpoint;
         const double obj = 1.0 / S;
@@ -1805,19 +1804,18 @@ int sm_adjoint(sm_ctx *c, int32_t type_a, int32_t type_b, int32_t nsteps, int32_
             c->pair_force();
             c->bonded(KICK_INIT, nullptr);
             cps[0].save(c);
-            double Jsum = 0.0;
+            std::vector<double> Jsum(R, 0.0), m(3 * (size_t)R);
             for (int s = 1; s <= S; s++) {
                 c->step_first(s);
                 c->step_finish(true, c->mc_step(s));
-                double m[3];
-                HIPCHK(hipMemcpyAsync(m, c->meas_dev(), sizeof m, hipMemcpyDeviceToHost, st));
+                HIPCHK(hipMemcpyAsync(m.data(), c->meas_dev(), sizeof(double) * m.size(), hipMemcpyDeviceToHost, st));
                 HIPCHK(hipStreamSynchronize(st));
-                Jsum += m[0] + m[1];
+                for (int r = 0; r < R; r++) Jsum[r] += m[3 * r] + m[3 * r + 1];
                 if (s % k == 0 && s / k < ncp) cps[s / k].save(c);
             }
             c->check_launch();
             c->check_flags();
-            *J = Jsum * obj;
+            for (int r = 0; r < R; r++) J[r] = Jsum[r] * obj;
 
             if (grad) {
                 segBeads = dalloc<Bead>((size_t)k * n);
@@ -1825,15 +1823,16 @@ int sm_adjoint(sm_ctx *c, int32_t type_a, int32_t type_b, int32_t nsteps, int32_
                 segPost = dalloc<DevRep>((size_t)k * c->nrep);
                 X = dalloc<double4>(n); V = dalloc<double4>(n); A = dalloc<double4>(n);
                 F = dalloc<double4>(n); HV = dalloc<double4>(n);
-                g = dalloc<double>(4 * (size_t)n);
-                gs = dalloc<double>(4);
+                g = dalloc<double>(kNTheta * (size_t)n);
+                gs = dalloc<double>(kNTheta * (size_t)R);
                 for (double4 *p : {X, V, A, F, HV}) HIPCHK(hipMemsetAsync(p, 0, sizeof(double4) * n, st));
-                HIPCHK(hipMemsetAsync(g, 0, sizeof(double) * 4 * n, st));
+                HIPCHK(hipMemsetAsync(g, 0, sizeof(double) * kNTheta * n, st));
                 const AdjTables T{c->adjF, c->adjDF, c->adjDU};
                 auto adjoint_forces = [&](double o, int hv) {
                     launch_adj_pair(st, c->P, c->sbeads, c->skey(), c->sidx(), c->cstart, c->cend, c->reps, T, A, F,
                                     HV, g, o, hv);
                     launch_adj_chain(st, c->P, c->beads, c->chainPerBead, c->cconstD, c->reps, A, F, HV, g, o, hv);
+                    if (c->gptr) launch_adj_gen(st, c->P, c->gen(), c->beads, c->reps, A, F, HV, hv);
                 };
                 for (int seg = ncp - 1; seg >= 0; seg--) {
                     const int s0 = seg * k, s1 = std::min(S, s0 + k);
@@ -1867,10 +1866,10 @@ int sm_adjoint(sm_ctx *c, int32_t type_a, int32_t type_b, int32_t nsteps, int32_
                 cps[0].load(c);
                 c->build(0, kStepKeep);
                 adjoint_forces(0.0, 0);
-                launch_seg_sum(st, c->segB, g, gs + 0, 1);
-                for (int p = 1; p < kNTheta; p++) launch_seg_sum(st, c->segB, g + (size_t)p * n, gs + p, 1);
+                // per replica: grad[r * kNTheta + p]
+                for (int p = 0; p < kNTheta; p++) launch_seg_sum(st, c->segB, g + (size_t)p * n, gs + p, kNTheta);
                 c->check_launch();
-                HIPCHK(hipMemcpyAsync(grad, gs, sizeof(double) * kNTheta, hipMemcpyDeviceToHost, st));
+                HIPCHK(hipMemcpyAsync(grad, gs, sizeof(double) * kNTheta * R, hipMemcpyDeviceToHost, st));
                 HIPCHK(hipStreamSynchronize(st));
             }
             entry.load(c);
