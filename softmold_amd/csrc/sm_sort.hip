@@ -7,8 +7,8 @@
 // with wave64 ballots, publishes its per-digit counts, accumulates the counts of the
 // tiles before it by decoupled look-back, and scatters.
 //
-// No kernel of its own clears or counts: the digit histograms of every pass are built by
-// the kernel that computes the keys (k_first), and the sort's scratch (histograms, tile
+// No kernel of its own clears or builds histograms: the digit histograms of every pass are
+// built by the kernel that computes the keys (k_first), and the sort's scratch (histograms, tile
 // counters, look-back states) is double-buffered -- while one sort uses scratch[p], the
 // reorder kernel after it clears scratch[1-p] for the next sort.
 //
@@ -22,9 +22,11 @@
 // change key.  The previous sorted order, with those beads taken out, is still sorted by
 // (key, index); the movers, compacted in index order and sorted by key with the same
 // stable passes, are sorted by (key, index) too; merging the two by (key, index) gives the
-// unique order that sorting all keys stably from index order gives.  k_compact lists the
-// movers and counts, per slot of the previous order, the movers before it; k_merge places
-// every stay and every mover by one binary search in the other sequence.
+// unique order that sorting all keys stably from index order gives.  k_count_moves and
+// k_compact list the movers and count, per slot of the previous order, the movers before
+// it; k_merge places every stay by a binary search among the sorted movers, and every
+// mover by a search among the previous slots (inside its cell's previous run when that
+// cell was occupied).
 #include <algorithm>
 
 #include "sm_sort_dev.hpp"
