@@ -250,3 +250,29 @@ def test_trajectory_tracks_oracle_with_same_noise(name, steps):
     print(f"{name}: max |dp| {np.abs(dp).max():.3e}  max |dv| {np.abs(vel - o.vel).max():.3e}")
     assert np.abs(dp).max() < 1e-4
     assert np.abs(vel - o.vel).max() < 1e-2
+
+
+def test_runaway_bead_is_reported_as_a_state_error():
+    """A bead given a velocity that carries it past the box's far side within one step (one
+    wrap does not bring it back) trips the key kernel's position check; every index stays
+    clamped inside the grid, the step completes, and the next synchronising call reports
+    SM_ERR_STATE.  A new context on the same device then runs normally."""
+    d = system("L500")
+    s = sm.Solver.from_system(d)
+    s.init_forces()
+    s.run(0, 4)
+    s.sync()
+    pos, vel, acc, box = s.download()
+    vel = vel.copy()
+    vel[17] = [1.0e5, 0.0, 0.0]   # 2000 per step at dt 0.02, five box lengths
+    s.upload(vel=vel)
+    s.run(4, 2)
+    with pytest.raises(sm.SmError, match="left the box") as e:
+        s.sync()
+    assert e.value.code == sm.SM_ERR_STATE
+    s.close()
+    t = sm.Solver.from_system(d)
+    t.init_forces()
+    t.run(0, 8)
+    t.sync()
+    assert np.all(np.isfinite(t.measure()))
