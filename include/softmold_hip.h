@@ -179,7 +179,9 @@ int sm_potential_delta(sm_ctx *ctx, const double *scale, double *du_pair, double
  * "sort_mode" = 1 to re-sort the cell keys incrementally -- the beads whose cell changed
  * are sorted and merged into the previous order (the default from a million beads up) --
  * or 0 to sort every key each build.  Both give the same order, so neither changes a
- * result. */
+ * result.  "mc_fused" = 1 (default) to take a box-move step's pair dU in the traversal of
+ * that step's pair force, 0 in a gather of its own after the kick (sm_potential_delta
+ * follows the same knob); the two sum the same FP64 terms in different orders. */
 int sm_tune(sm_ctx *ctx, const char *knob, int32_t value);
 int sm_tune_get(sm_ctx *ctx, const char *knob, int32_t *value);
 /* Autotuner (no reference counterpart): times `steps` steps of this context's own work

@@ -197,6 +197,9 @@ struct sm_ctx {
     unsigned long long *dnp = nullptr;
     int chunk = 8;
     int pair_variant = 0;
+    // box-move steps: the pair dU in the same traversal as the step's pair force (1) or in
+    // a gather of its own after the kick (0, as the adjoint uses)
+    bool mc_fused = true;
     int sorted_sel = 0;   // which sort buffer pair holds the sorted result
     // captured 8-step graph
     bool use_graph = false;
@@ -466,6 +469,11 @@ struct sm_ctx {
                                nullptr, nullptr, nullptr, 0);
             launch_seg_sum(st, segB, ebead, du_dev() + 0, 2);
         });
+        delta_bonded();
+    }
+    // the CHAIN and generic-record part of the dU
+    void delta_bonded()
+    {
         timed(K_CHAIN_E, [&] {
             launch_chain_energy(st, nchains, chainList, cconstD, beads, reps, 1, echain);
             launch_seg_sum(st, segC, echain, du_dev() + 1, 2);
@@ -479,10 +487,48 @@ struct sm_ctx {
     {
         timed(K_MC, [&] { launch_mc_propose(st, nrep, reps, glob); });
         delta_device();
+        mc_decide();
+    }
+    void mc_decide()
+    {
         timed(K_MC, [&] {
             launch_mc_decide(st, P, reps, glob, du_dev());
             launch_rescale(st, n, beads, reps);
         });
+    }
+    // A box-move step's pair force with its pair dU: the trial scale is drawn first (it
+    // depends on the step's key only), then one traversal gives the forces and each owned
+    // pair's U(old) - U(scaled) at the positions the force sees, which the kick leaves as
+    // they are.  The bonded dU and the decision follow the kick (mc_finish).
+    void pair_force_mc()
+    {
+        timed(K_MC, [&] { launch_mc_propose(st, nrep, reps, glob); });
+        pair_force_du();
+    }
+    // the pair force with the pair dU under each replica's current trial scale
+    void pair_force_du()
+    {
+        timed(K_PAIR, [&] {
+            launch_pair_force_du(st, P, chunk, sbeads, skey(), sidx(), cstart, cend, reps, fconstF, uconstD, fpair,
+                                 ebead);
+            launch_seg_sum(st, segB, ebead, du_dev() + 0, 2);
+        });
+    }
+    void pair_step(bool mc)
+    {
+        if (mc && mc_fused)
+            pair_force_mc();
+        else
+            pair_force();
+    }
+    void mc_finish()
+    {
+        if (mc_fused) {
+            delta_bonded();
+            mc_decide();
+        } else {
+            mc_move();
+        }
     }
     bool mc_step(long long step) const { return step % 8 == 0 && step != 0 && any_mc; }
 
@@ -507,10 +553,10 @@ struct sm_ctx {
     void step_finish(bool measure, bool mc)
     {
         sort_and_reorder();
-        pair_force();
+        pair_step(mc);
         bonded(KICK_STEP, nullptr);
         if (measure) measure_device();
-        if (mc) mc_move();
+        if (mc) mc_finish();
     }
     void eager_step(long long step)
     {
@@ -573,13 +619,13 @@ struct sm_ctx {
             if (!first_done) step_first(s);
             const bool mc = mc_step(s);
             sort_and_reorder();
-            pair_force();
+            pair_step(mc);
             if (!mc && s + 1 < end) {
                 kick_first(s);
                 first_done = true;
             } else {
                 bonded(KICK_STEP, nullptr);
-                if (mc) mc_move();
+                if (mc) mc_finish();
                 first_done = false;
             }
         }
@@ -1379,7 +1425,12 @@ int sm_potential_delta(sm_ctx *c, const double *scale, double *du_pair, double *
             for (int d = 0; d < 3; d++) hr[r].scale[d] = scale[3 * r + d];
         c->write_reps(hr);
         c->build(0, kStepKeep);
-        c->delta_device();
+        if (c->mc_fused) {   // the traversal a box-move step runs (it also leaves the pair forces)
+            c->pair_force_du();
+            c->delta_bonded();
+        } else {
+            c->delta_device();
+        }
         c->check_launch();
         std::vector<double> h(2 * (size_t)c->nrep);
         HIPCHK(hipMemcpyAsync(h.data(), c->du_dev(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->st));
@@ -1409,6 +1460,9 @@ int sm_tune(sm_ctx *c, const char *knob, int32_t value)
             if (value < 0 || value > 1) invalid("sort_mode must be 0 or 1");
             c->sort_mode = value;
             c->order_valid = false;
+        } else if (std::strcmp(knob, "mc_fused") == 0) {
+            if (value < 0 || value > 1) invalid("mc_fused must be 0 or 1");
+            c->mc_fused = value != 0;
         } else {
             invalid(std::string("unknown knob ") + knob);
         }
@@ -1424,6 +1478,7 @@ int sm_tune_get(sm_ctx *c, const char *knob, int32_t *value)
         else if (std::strcmp(knob, "pair_variant") == 0) *value = c->pair_variant;
         else if (std::strcmp(knob, "graph") == 0) *value = c->use_graph ? 1 : 0;
         else if (std::strcmp(knob, "sort_mode") == 0) *value = c->sort_mode;
+        else if (std::strcmp(knob, "mc_fused") == 0) *value = c->mc_fused ? 1 : 0;
         else invalid(std::string("unknown knob ") + knob);
     });
 }
@@ -1781,6 +1836,14 @@ int sm_adjoint(sm_ctx *c, int32_t type_a, int32_t type_b, int32_t nsteps, int32_
         upload_adjoint_tables(c, type_a, type_b);
 
         const int ncp = (S + k - 1) / k;   // checkpoints at steps 0, k, 2k, ...
+        // the forward run and the recomputed segments take their box-move dU the same way
+        // (the gather after the kick), so they make the same decisions
+        struct KeepFused {
+            sm_ctx *c;
+            bool v;
+            ~KeepFused() { c->mc_fused = v; }
+        } keep{c, c->mc_fused};
+        c->mc_fused = false;
         StateCopy entry;
         std::vector<StateCopy> cps(ncp);
         Bead *segBeads = nullptr;

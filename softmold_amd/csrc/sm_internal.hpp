@@ -127,6 +127,12 @@ void launch_reorder(hipStream_t st, int n, const uint32_t *skey, const uint32_t 
 void launch_pair_force(hipStream_t st, const Params &P, int chunk, int variant, const Bead *sbeads,
                        const uint32_t *skey, const uint32_t *sidx, const int *cstart, const int *cend,
                        const DevRep *reps, const float *tab, float4 *fpair);
+// The one-phase pair force of a box-move step, fused with the FP64 dU gather: fpair as
+// launch_pair_force writes it, and energy[] per sorted slot as GATHER_DU does, under each
+// replica's trial scale (set by launch_mc_propose before).
+void launch_pair_force_du(hipStream_t st, const Params &P, int chunk, const Bead *sbeads, const uint32_t *skey,
+                          const uint32_t *sidx, const int *cstart, const int *cend, const DevRep *reps,
+                          const float *fconst, const double *uconst, float4 *fpair, double *energy);
 // gptr == NULL: CHAIN terms only; else also the generic records of each bead
 struct GenTables {
     const int *gptr;

@@ -42,7 +42,7 @@ namespace {
 constexpr int kWaves = 1;   // waves per workgroup: one, so a finished wave frees its slot at once
 constexpr int kList = 32;      // candidates a lane screens before evaluating its hits
 
-enum : int { M_FORCE = 0, M_U = 1, M_DU = 2, M_COUNT = 3, M_LIST = 4, M_FORCE1 = 5, M_FCOUNT = 6 };
+enum : int { M_FORCE = 0, M_U = 1, M_DU = 2, M_COUNT = 3, M_LIST = 4, M_FORCE1 = 5, M_FCOUNT = 6, M_FDU = 7 };
 
 struct __align__(16) Cand {
     double x, y, z;
@@ -200,6 +200,42 @@ __device__ inline Cand load_cand(const Cand *p)
     return c;
 }
 
+// One owned pair's U(old) - U(scaled) from its imaged FP64 difference (dx, dy, dz) and its
+// potential row C.  Under the trial scale every coordinate difference scales by s_k, so the
+// scaled r^2 is r^2 + sum (s_k^2 - 1) d_k^2 from the differences at hand (s2 = s_k^2 - 1);
+// it differs from differencing the scaled positions (cellOpt.h:1098-1105) by a rounding,
+// where the potential is continuous.  dseries: every |s_k^2 - 1| <= 1e-2, so r' - r comes
+// from the binomial series of sqrt(1 + delta).
+__device__ inline double du_pair(double dx, double dy, double dz, double r2, const double *C, double cut2,
+                                 double s2x, double s2y, double s2z, bool dseries)
+{
+    const double dr2 = fma(s2x, dx * dx, fma(s2y, dy * dy, s2z * (dz * dz)));
+    const double r2n = r2 + dr2;
+    const bool in0 = r2 < cut2, in1 = r2n < cut2;
+    if (!(in0 || in1)) return 0.0;
+    if (!dseries) return (in0 ? pair_u_any(r2, C) : 0.0) - (in1 ? pair_u_any(r2n, C) : 0.0);
+    // r' - r = r (sqrt(1 + delta) - 1) by the binomial series through delta^5: with
+    // |delta| <= 1e-2 the rest is below 5e-12 of dr
+    double r, ri;
+    sqrt_inv_pos(r2, r, ri);
+    const double del = dr2 * (ri * ri);
+    const double dr = (r * del) * fma(del, fma(del, fma(del, fma(del, 0.02734375, -0.0390625), 0.0625), -0.125), 0.5);
+    const double rn = r + dr;
+    const bool innerO = r <= C[0], innerN = rn <= C[0];
+    if (in0 && in1 && innerO == innerN) {
+        // both distances on one polynomial branch: U(r) - U(r + dr) in closed form, with no
+        // cancellation between the two potentials
+        const double a = C[0] - r;
+        const double uin = C[1] * dr * fma(2.0, a, -dr);
+        const double b = C[3] - r;
+        const double bn = b - dr;
+        const double uout = dr * (C[4] * fma(2.0, b, -dr) - C[5] * fma(3.0 * b, bn, dr * dr));
+        return innerO ? uin : uout;
+    }
+    // a branch or the cutoff lies between the two distances
+    return (in0 ? pair_u_r(r, C) : 0.0) - (in1 ? pair_u_r(rn, C) : 0.0);
+}
+
 // One candidate of the force kernels: the exact FP64 test r^2 < rc^2 (MD.h:798-810), then
 // the force law (MD.h:811-848) in FP32, its magnitude times the FP64 difference added to
 // FP64 sums.  With r0 = r2f * rsq(r2f) (exact as an FP64
@@ -314,9 +350,13 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     const int *__restrict__ cend, const DevRep *__restrict__ reps, const float *__restrict__ tab,
     const double *__restrict__ uconst, Out out)
 {
-    constexpr bool kF = MODE == M_FORCE || MODE == M_FORCE1;   // force (two-phase / one-phase)
+    // the one-phase force of a box-move step with the FP64 U(old) - U(scaled) of each pair
+    // the home particle owns, in the same traversal (the box move's dU at the positions the
+    // force sees)
+    constexpr bool kFDU = MODE == M_FDU;
+    constexpr bool kF = MODE == M_FORCE || MODE == M_FORCE1 || kFDU;   // force (two-phase / one-phase)
     constexpr bool k2P = MODE == M_FORCE;
-    constexpr bool kF1 = MODE == M_FORCE1;
+    constexpr bool kF1 = MODE == M_FORCE1 || kFDU;
     constexpr bool kE = MODE == M_U || MODE == M_DU;   // FP64 energy gathers
     // the one-phase force traversal counting its accepted pairs with a nonzero term
     // instead of evaluating them: same staging, culling, lists and FP64 test
@@ -325,11 +365,11 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     // staged candidates per pass: the force kernel measured fastest at 192; the two-phase
     // force and the FP64 energy gathers at 128 (their larger constant tables and registers
     // leave less LDS per resident wave)
-    constexpr int kMaxC = (k2P || kE) ? 128 : 192;
+    constexpr int kMaxC = (k2P || kE || kFDU) ? 128 : 192;
     constexpr int kStageIters = kMaxC / kWave;
     static_assert(kMaxC % kWave == 0, "staging rounds cover whole waves");
     __shared__ float4 sC8[kF ? 2 * kMaxTypes * kMaxTypes : 1];   // rows padded to 8 floats
-    __shared__ double sCd[kF ? 1 : 6 * kMaxTypes * kMaxTypes];
+    __shared__ double sCd[(kF && !kFDU) ? 1 : 6 * kMaxTypes * kMaxTypes];
     __shared__ Cand sCand[kWaves][kMaxC];   // staged candidates of the current pass
     __shared__ float4 sRel[k2P ? kWaves : 1][k2P ? kMaxC : 1];
     __shared__ uint8_t sList[k2P ? kWaves : 1][k2P ? kList : 1][kWave];   // per-lane hit lists, lane-minor
@@ -356,10 +396,14 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     if (kF) {
         const float4 *t8 = reinterpret_cast<const float4 *>(tab);
         for (int t = threadIdx.x; t < 2 * nT * nT; t += blockDim.x) sC8[t] = t8[t];
-    } else {
+    }
+    if (!kF || kFDU) {
         for (int t = threadIdx.x; t < 6 * nT * nT; t += blockDim.x) sCd[t] = uconst[t];
     }
-    if (kCull && (int)threadIdx.x < nT) sReach2[threadIdx.x] = tab[8 * nT * nT + (kE ? kMaxTypes : 0) + threadIdx.x];
+    if (kCull && (int)threadIdx.x < nT) {
+        const float rf = tab[8 * nT * nT + threadIdx.x], re = tab[8 * nT * nT + kMaxTypes + threadIdx.x];
+        sReach2[threadIdx.x] = kFDU ? fmaxf(rf, re) : (kE ? re : rf);
+    }
     __syncthreads();
 
     const int lane = threadIdx.x & (kWave - 1);
@@ -399,7 +443,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
             // energy gathers: a candidate farther than (rc + margin) / min(1, scale) from the
             // home quarter-cell is beyond rc both before and after the trial scaling
             const double smin = fmin(1.0, fmin(ax, fmin(ay, az)));
-            const float reachS = MODE == M_DU ? (float)(1.0 / (smin * smin)) : 1.f;   // dU: widened by the trial scale
+            const float reachS = (MODE == M_DU || kFDU) ? (float)(1.0 / (smin * smin)) : 1.f;   // dU: widened by the trial scale
             const uint32_t kbase = (uint32_t)R.cellbase;
             const uint32_t k = kg - kbase;
             const int kx = (int)(k % (uint32_t)nx);
@@ -533,7 +577,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         // count and list gathers, and the force over an asymmetric table: the
                         // slot, with the stencil cell above the type's bits (types are below
                         // kMaxTypes <= 256)
-                        constexpr bool kSlotCell = MODE == M_COUNT || MODE == M_LIST || (ASYM && (kF || kFC));
+                        constexpr bool kSlotCell = MODE == M_COUNT || MODE == M_LIST || kFDU || (ASYM && (kF || kFC));
                         cd.t = kSlotCell ? (qb[it].type | (c << 8)) : qb[it].type;
                         cd.c = (kF || kFC || kSlotCell) ? src[it] : c;
                         Cw[lane + it * kWave] = cd;
@@ -554,7 +598,29 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         }
                     }
                     wave_lds_sync();
-                    if ((MODE == M_FORCE1 || kFC) && act) {
+                    if (kFDU && act) {
+                        // the force of every candidate and, for each pair this side owns
+                        // (forward half shell; the higher slot within the home cell), the
+                        // FP64 dU term with the row of the particle the reference makes first
+                        const int quad = (hfx >= 0.f ? 1 : 0) + (hfy >= 0.f ? 2 : 0);
+                        const uint8_t *lst = mapW + quad * kMaxC;
+                        const int nl = quad == 0 ? nq[0] : (quad == 1 ? nq[1] : (quad == 2 ? nq[2] : nq[3]));
+                        for (int jj = sub; jj < nl; jj += tpp) {
+                            const Cand q = load_cand(Cw + lst[jj]);
+                            const int tq = q.t & 0xff, c = q.t >> 8;
+                            const double dxd = hb.x - q.x, dyd = hb.y - q.y, dzd = hb.z - q.z;
+                            const double r2 = sq3(dxd, dyd, dzd);
+                            // the row of the pair's first particle (home_first; symmetric
+                            // tables take the home's row either way)
+                            bool hf = true;
+                            if (ASYM) hf = c == 13 ? me < sidx[q.c] : ((fwd >> c) & 1u) != 0;
+                            const int row = hf ? hb.type * nT + tq : tq * nT + hb.type;
+                            if (r2 < cut2 && q.c != hslot) force_law(dxd, dyd, dzd, r2, sC8 + 2 * row, fx, fy, fz);
+                            // owned: a forward half-shell cell, or the higher slot in the home cell
+                            const bool owned = c == 13 ? q.c > hslot : ((fwd >> c) & 1u) != 0;
+                            if (owned) accu += du_pair(dxd, dyd, dzd, r2, sCd + 6 * row, cut2, s2x, s2y, s2z, dseries);
+                        }
+                    } else if ((MODE == M_FORCE1 || kFC) && act) {
                         // one phase over the home's quarter list: FP64 test, force law under
                         // the branch; two candidates per trip so both LDS reads are in
                         // flight before either is used
@@ -634,11 +700,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         const uint8_t *lst = mapW + quad * kMaxC;
                         const int nl = quad == 0 ? nq[0] : (quad == 1 ? nq[1] : (quad == 2 ? nq[2] : nq[3]));
                         const double *Ch = sCd + 6 * hb.type * nT;
-                        // one owned pair's share: U, or U(old) - U(scaled).  Under the trial
-                        // scale every coordinate difference scales by s_k, so the scaled r^2 is
-                        // r^2 + sum (s_k^2 - 1) d_k^2 from the differences already at hand; it
-                        // differs from differencing the scaled positions (cellOpt.h:1098-1105)
-                        // by a rounding, where the potential is continuous
+                        // one owned pair's share: U, or U(old) - U(scaled) (du_pair)
                         auto term = [&](const Cand &q, int j) -> double {
                             if (j == self || (q.c == 13 && j < self)) return 0.0;
                             // the staged forward half shell's pairs have the home particle
@@ -650,33 +712,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                             const double dx = hb.x - q.x, dy = hb.y - q.y, dz = hb.z - q.z;   // staged imaged
                             const double r2 = sq3(dx, dy, dz);
                             if (MODE == M_U) return r2 < cut2 ? pair_u_any(r2, C) : 0.0;
-                            const double dr2 = fma(s2x, dx * dx, fma(s2y, dy * dy, s2z * (dz * dz)));
-                            const double r2n = r2 + dr2;
-                            const bool in0 = r2 < cut2, in1 = r2n < cut2;
-                            if (!(in0 || in1)) return 0.0;
-                            if (!dseries) return (in0 ? pair_u_any(r2, C) : 0.0) - (in1 ? pair_u_any(r2n, C) : 0.0);
-                            // r' - r = r (sqrt(1 + delta) - 1) by the binomial series through
-                            // delta^5: with |delta| <= 1e-2 the rest is below 5e-12 of dr
-                            double r, ri;
-                            sqrt_inv_pos(r2, r, ri);
-                            const double del = dr2 * (ri * ri);
-                            const double dr =
-                                (r * del) *
-                                fma(del, fma(del, fma(del, fma(del, 0.02734375, -0.0390625), 0.0625), -0.125), 0.5);
-                            const double rn = r + dr;
-                            const bool innerO = r <= C[0], innerN = rn <= C[0];
-                            if (in0 && in1 && innerO == innerN) {
-                                // both distances on one polynomial branch: U(r) - U(r + dr) in
-                                // closed form, with no cancellation between the two potentials
-                                const double a = C[0] - r;
-                                const double uin = C[1] * dr * fma(2.0, a, -dr);
-                                const double b = C[3] - r;
-                                const double bn = b - dr;
-                                const double uout = dr * (C[4] * fma(2.0, b, -dr) - C[5] * fma(3.0 * b, bn, dr * dr));
-                                return innerO ? uin : uout;
-                            }
-                            // a branch or the cutoff lies between the two distances
-                            return (in0 ? pair_u_r(r, C) : 0.0) - (in1 ? pair_u_r(rn, C) : 0.0);
+                            return du_pair(dx, dy, dz, r2, C, cut2, s2x, s2y, s2z, dseries);
                         };
                         for (int jj = sub; jj < nl; jj += tpp) {
                             const int i0 = lst[jj];
@@ -729,6 +765,13 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         }
                     }
                     if (act && sub == 0) out.force[me] = make_float4((float)fx, (float)fy, (float)fz, 0.f);
+                    if (kFDU) {
+                        for (int st = 1; st < tpp; st <<= 1) {
+                            const double o = __shfl(accu, lane + st);
+                            if ((sub & (2 * st - 1)) == 0 && sub + st < tpp) accu += o;
+                        }
+                        if (act && sub == 0) out.energy[s0 + h0 + hi] = accu;
+                    }
                 } else if (MODE == M_COUNT || kFC) {
                     for (int st = 1; st < tpp; st <<= 1) {
                         const int o = __shfl(cnt, lane + st);
@@ -788,6 +831,17 @@ void launch_pair_force(hipStream_t st, const Params &P, int chunk, int variant, 
         launch_cells<M_FORCE1, true>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, fconst, nullptr, o);
     else
         launch_cells<M_FORCE1>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, fconst, nullptr, o);
+}
+
+void launch_pair_force_du(hipStream_t st, const Params &P, int chunk, const Bead *sbeads, const uint32_t *skey,
+                          const uint32_t *sidx, const int *cstart, const int *cend, const DevRep *reps,
+                          const float *fconst, const double *uconst, float4 *fpair, double *energy)
+{
+    Out o{fpair, energy, nullptr, nullptr, nullptr, 0};
+    if (P.asym)
+        launch_cells<M_FDU, true>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, fconst, uconst, o);
+    else
+        launch_cells<M_FDU>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, fconst, uconst, o);
 }
 
 void launch_pair_gather(hipStream_t st, const Params &P, int chunk, int mode, const Bead *sbeads,

@@ -161,10 +161,14 @@ def test_cfg1_known_answer():
     assert K == pytest.approx(1.5 * 15000 * 3.0, rel=1e-6)
 
 
+@pytest.mark.parametrize("fused", [1, 0])
 @pytest.mark.parametrize("name", ["B400", "B4000", "L5000", "B1M"])
-def test_potential_delta_fp64(name):
+def test_potential_delta_fp64(name, fused):
+    """dU of the pair and CHAIN terms under two trial scales, from the traversal fused with
+    the pair force (what a box-move step runs) and from the dU gather."""
     d = system(name)
     s = sm.Solver.from_system(d)
+    s.tune("mc_fused", fused)
     o = oracle_of(d)
     for scale in ([1.0001, 1.0001, 1 / 1.0001 ** 2], [0.999, 0.999, 1 / 0.999 ** 2]):
         dp, dc = s.potential_delta(scale)
@@ -276,3 +280,27 @@ def test_runaway_bead_is_reported_as_a_state_error():
     t.run(0, 8)
     t.sync()
     assert np.all(np.isfinite(t.measure()))
+
+
+@pytest.mark.parametrize("name", ["B4000", "B400"])
+def test_fused_box_move_dU_keeps_the_trajectory(name):
+    """Box moves every 8 steps (deltaLXY 0.05, tension 1) with the pair dU taken in the
+    step's force traversal and in a gather after the kick: the same decisions and the same
+    trajectory, bit for bit (dU decides only the accept bit), eager and in captured graphs."""
+    d = dict(system(name), delta_lxy=0.05, tension=1.0, seed=11)
+    print(f"case fused dU {name}", flush=True)
+    a = sm.Solver.from_system(d)
+    b = sm.Solver.from_system(d)
+    b.tune("mc_fused", 0)
+    for s in (a, b):
+        s.init_forces()
+        s.run(0, 120)
+        s.tune("graph", 1)
+        s.run(120, 200)
+    assert a.mc_stats() == b.mc_stats() and a.mc_stats()[0] == 39
+    for x, y in zip(a.download(), b.download()):
+        assert np.array_equal(x, y)
+    for r in range(2):
+        ha, hb = a.mc_hist(), b.mc_hist()
+        assert np.array_equal(ha[r], hb[r])
+
