@@ -154,7 +154,7 @@ def test_energies_fp64(name):
     assert K == pytest.approx(o.kinetic(), rel=1e-6)
 
 
-def test_cfg1_known_answer():
+def test_5000_lipid_vesicle_known_answer():
     s = sm.Solver.from_system(system("L5000"))
     U_pair, _, K = s.measure()
     assert U_pair == pytest.approx(-388321.0139620534, rel=1e-10)
