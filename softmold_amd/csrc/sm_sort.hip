@@ -56,7 +56,7 @@ __global__ __launch_bounds__(kThreads) void k_onesweep(int n, const int *__restr
                                                        int *__restrict__ errflag)
 {
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t scan[256];
+    __shared__ uint32_t scan[kWavesPerBlock];
     // per (round, wave, digit): item count, then its exclusive output position
     __shared__ uint32_t wc[kRounds][kWavesPerBlock][256];
     const int t = threadIdx.x;
@@ -64,19 +64,24 @@ __global__ __launch_bounds__(kThreads) void k_onesweep(int n, const int *__restr
     const bool many = ndev != nullptr;
     if (many) n = *ndev;
     if (t == 0) s_tile = atomicAdd(tilectr, 1u);
-    // digit base: keys of smaller digits over all keys, from every histogram copy
+    // digit base: keys of smaller digits over all keys, from every histogram copy; a scan
+    // inside each wave by shuffles, then the waves' totals (one barrier instead of the 16 of
+    // an LDS scan: 37.2 against 38.3 us per 240k-key sort)
     uint32_t g = 0;
 #pragma unroll
     for (int c = 0; c < kHistCopies; c++) g += ghist[(size_t)c * 4 * 256 + t];
-    scan[t] = g;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        const uint32_t v = (t >= off) ? scan[t - off] : 0u;
-        __syncthreads();
-        scan[t] += v;
-        __syncthreads();
+    const int lane = t & (kWave - 1);
+    uint32_t x = g;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
     }
-    const uint32_t dbase = scan[t] - g;
+    if (lane == kWave - 1) scan[wave] = x;
+    __syncthreads();
+    uint32_t wpre = 0;
+    for (int w = 0; w < wave; w++) wpre += scan[w];
+    const uint32_t dbase = wpre + x - g;
     const unsigned long long lt = lanes_below();
     for (;;) {
         for (int r = 0; r < kRounds; r++)
