@@ -224,3 +224,31 @@ def test_asymmetric_adjoint_matches_fp64_differences(golden_traj):
     print(f"J gpu {J:.10g} oracle {J64:.10g}\nadjoint {g}\nFP64 differences {fd64}")
     assert J == pytest.approx(J64, rel=1e-6)
     np.testing.assert_allclose(g, fd64, rtol=1e-3)
+
+
+@pytest.mark.gpu
+def test_asymmetric_md_hip_step0_matches_reference_md(golden_traj, tmp_path):
+    """The drop-in executable on the reference-written AB400 input (asymmetric head-tail
+    rows, tension 1): its step-0 potential_ and kinetic_ lines equal the compiled reference
+    MD's, as MD prints them (6 significant digits).  Later lines differ by design: the
+    noise comes from a counter RNG."""
+    if not gpu_available():
+        pytest.skip("no GPU")
+    import subprocess
+
+    cli = os.path.join(os.path.dirname(sm.__file__), "md_hip")
+    name = "AB400t"
+    (tmp_path / f"{name}.mpd").write_bytes(golden_traj[name + "/input_mpd"].tobytes())
+    print("case: md_hip on AB400t", flush=True)
+    r = subprocess.run([cli, name], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+
+    def first(path):
+        with open(path) as f:
+            return [float(x) for x in f.readline().split("\t")]
+
+    for dat in ("potential", "kinetic"):
+        got = first(tmp_path / f"{dat}_{name}.dat")
+        want = golden_traj[f"{name}/{dat}"][0]
+        assert got[0] == want[0] == 0.0
+        assert float(f"{got[1]:.6g}") == want[1], (dat, got, want)
