@@ -282,6 +282,8 @@ def main():
     ap.add_argument("--calib-copy", action="store_true")
     ap.add_argument("--batched-only", action="store_true", help="only the replica sweep (for profiling it)")
     ap.add_argument("--bilayer-only", action="store_true", help="only the 1M-lipid flat bilayer (for profiling it)")
+    ap.add_argument("--sort-mode", type=int, choices=(0, 1), default=None,
+                    help="the plain workload's cell sort: 0 every key, 1 incremental (default: the solver's)")
     args = ap.parse_args()
     if args.calib_copy:
         calib_copy()
@@ -306,6 +308,8 @@ def main():
     d = sm.Mpd.liposome(SEED, NLIPIDS, DENSITY).system()
     n = len(d["type"])
     s = sm.Solver.from_system(d)
+    if args.sort_mode is not None:
+        s.tune("sort_mode", args.sort_mode)
     keys = s.cell_keys()
     nc = (d["box"] / d["cutoff"]).astype(int)
     occ, cand = stencil_stats(keys, nc)
