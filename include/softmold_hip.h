@@ -111,7 +111,10 @@ int sm_step_first(sm_ctx *ctx, int64_t step);
 /* Rest of step `step` (MD.cpp:410-511, 589-721): Langevin, cell build, pair and CHAIN forces,
  * Verlet::second; if measure != NULL it receives {U_pair, U_chain, K} of each replica after
  * the kick (dataExtraction.h:827-1050); then the MC box move when step%8==0, step!=0, in
- * every replica with delta_lxy != 0. */
+ * every replica with delta_lxy != 0. With the "mc_fused" knob on (the default) a box-move
+ * step draws its trial scale before the pair force and takes the pair dU in the same
+ * traversal. Both paths sum the same FP64 pair terms, in different orders; the knob set to
+ * 0 gives the separate dU gather. */
 int sm_step_finish(sm_ctx *ctx, int64_t step, double *measure);
 /* Whole steps first_step .. first_step+nsteps-1 with no measurement, all on the device.
  * With the "graph" knob on, aligned blocks of 8 steps replay one captured hipGraph. */
