@@ -168,3 +168,34 @@ def test_autotune_restores_state_and_keeps_results():
     for x, y in zip(_state(a), _state(b)):
         assert np.array_equal(x, y)
     assert np.array_equal(a.measure(), b.measure())
+
+
+def test_knobs_validate_and_round_trip():
+    """sm_tune / sm_tune_get: every knob reads back what was set, and an unknown knob or a
+    value outside its range is refused with SM_ERR_INVALID, leaving the knob unchanged."""
+    if not gpu_available():
+        pytest.skip("no GPU")
+    s = sm.Solver.from_system(sm.Mpd.bilayer(1234, 250, 3.45).system())
+    valid = {"pair_chunk": (1, 8, 4096), "pair_variant": (0, 1), "graph": (0, 1), "sort_mode": (0, 1),
+             "mc_fused": (0, 1)}
+    invalid = {"pair_chunk": (0, -1, 4097), "pair_variant": (-1, 2), "graph": (2,), "sort_mode": (2,),
+               "mc_fused": (-1, 2)}
+    for knob, values in valid.items():
+        for v in values:
+            s.tune(knob, v)
+            assert s.tune_get(knob) == v, knob
+        kept = s.tune_get(knob)
+        for v in invalid[knob]:
+            with pytest.raises(sm.SmError) as err:
+                s.tune(knob, v)
+            assert err.value.code == sm.SM_ERR_INVALID and knob in str(err.value)
+            assert s.tune_get(knob) == kept
+    for call in (lambda: s.tune("no_such_knob", 1), lambda: s.tune_get("no_such_knob")):
+        with pytest.raises(sm.SmError) as err:
+            call()
+        assert err.value.code == sm.SM_ERR_INVALID and "unknown knob" in str(err.value)
+    # the context still steps after the refusals
+    s.tune("pair_chunk", 8)
+    s.init_forces()
+    s.run(0, 8)
+    assert np.all(np.isfinite(s.download()[0]))

@@ -304,3 +304,41 @@ def test_fused_box_move_dU_keeps_the_trajectory(name):
         ha, hb = a.mc_hist(), b.mc_hist()
         assert np.array_equal(ha[r], hb[r])
 
+
+
+@pytest.mark.parametrize("sort_mode", [0, 1])
+def test_systems_smaller_than_a_wave(sort_mode):
+    """One bead, two beads, and one 3-bead lipid chain beside a free bead, in a 10^3 box:
+    fewer beads than a wave, a key sort of one partial tile, and pair-kernel chunks past
+    the last bead.  Step-0 forces against the oracle, then 24 steps of the same noise."""
+    base = system("L500")
+    cases = {
+        "one": (np.array([[5.0, 5.0, 5.0]]), np.array([3], np.int32), np.zeros((0, 3), np.int32)),
+        "two": (np.array([[5.0, 5.0, 5.0], [5.6, 5.3, 4.8]]), np.array([3, 2], np.int32),
+                np.zeros((0, 3), np.int32)),
+        "chain": (np.array([[1.0, 9.5, 0.2], [1.5, 9.8, 9.9], [2.1, 0.1, 9.7], [1.2, 9.2, 0.9]]),
+                  np.array([2, 3, 3, 3], np.int32), np.array([[0, 1, 3]], np.int32)),
+    }
+    for name, (pos, types, chain) in cases.items():
+        n = len(types)
+        d = dict(base)
+        d.update(pos=pos, type=types, vel=np.zeros((n, 3)), box=np.array([10.0, 10.0, 10.0]), chain=chain,
+                 chain_const=np.array(base["chain_const"])[: len(chain)])
+        print(f"case tiny {name} n={n} sort_mode {sort_mode}", flush=True)
+        s = sm.Solver.from_system(d)
+        s.tune("sort_mode", sort_mode)
+        o = oracle_of(d)
+        np.testing.assert_array_equal(s.cell_keys(), o.cell_keys())
+        fp, _ = s.compute_forces()
+        ref = o.pair_force()
+        assert np.all(np.abs(fp - ref) <= 1e-5 * np.maximum(np.abs(ref), 1.0)), (fp, ref)
+        md = O.MD(o, rng_mode=1)
+        md.init_forces()
+        md.run(0, 23)
+        s.init_forces()
+        s.run(0, 24)
+        p, v, _, _ = s.download()
+        dp = p - o.pos
+        dp -= 10.0 * np.round(dp / 10.0)
+        print(f"{name}: max |dp| {np.abs(dp).max():.3e} max |dv| {np.abs(v - o.vel).max():.3e}")
+        assert np.abs(dp).max() < 1e-6 and np.abs(v - o.vel).max() < 1e-4
