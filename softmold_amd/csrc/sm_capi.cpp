@@ -671,6 +671,18 @@ void validate(const sm_system *s)
                         ": r_m below half the cutoff (the reference's branch offset int(r/r_m)*3 leaves the row)");
     if (s->n_chain_blocks < 0) invalid("negative chain block count");
     if (s->n_chain_blocks > 0 && (!s->chain || !s->chain_const)) invalid("chain blocks without data");
+    // CHAIN blocks (start, nChains, length): inside the particles, each bead in one chain
+    // at most (the reference checks its chains' bonds against nParticles, system.h:515-535)
+    std::vector<char> chained(s->n_chain_blocks > 0 ? n : 0, 0);
+    for (int b = 0; b < s->n_chain_blocks; b++) {
+        const int start = s->chain[3 * b], cnt = s->chain[3 * b + 1], len = s->chain[3 * b + 2];
+        if (len < 3 || len > 0xFFFF) unsupported("CHAIN length must be in [3, 65535]");
+        if (cnt < 0 || start < 0 || (long long)start + (long long)cnt * len > n) invalid("CHAIN block out of range");
+        for (int i = start; i < start + cnt * len; i++) {
+            if (chained[i]) unsupported("a bead belongs to two CHAIN records");
+            chained[i] = 1;
+        }
+    }
     if (s->n_bonds < 0 || s->n_bends < 0 || s->n_balls < 0) invalid("negative BOND / BEND / BALL record count");
     if ((s->n_bonds > 0 && (!s->bond || !s->bond_const)) || (s->n_bends > 0 && (!s->bend || !s->bend_const)) ||
         (s->n_balls > 0 && (!s->ball || !s->ball_const)))
@@ -690,10 +702,11 @@ void validate(const sm_system *s)
 // replicas of one context share the interaction constants, cutoff and time step
 void validate_batch(const sm_system *s, int nrep)
 {
-    long long total = 0;
+    long long total = 0, blocks = 0;
     for (int r = 0; r < nrep; r++) {
         validate(&s[r]);
         total += s[r].n_particles;
+        blocks += s[r].n_chain_blocks;
         if (r == 0) continue;
         const sm_system &a = s[0], &b = s[r];
         if (b.n_types != a.n_types || b.cutoff != a.cutoff || b.dt != a.dt)
@@ -704,6 +717,7 @@ void validate_batch(const sm_system *s, int nrep)
             unsupported("replicas must share the two-body constant tables");
     }
     if (total > 0x7FFFFFFF) unsupported("more than 2^31-1 beads in one context");
+    if (blocks > 0x8000) unsupported("too many CHAIN constant sets");
 }
 
 sm_ctx *create(const sm_system *s, int nrep)
@@ -736,24 +750,18 @@ sm_ctx *create(const sm_system *s, int nrep)
         std::vector<int2> list;
         std::vector<double4> ccD;
         for (int r = 0; r < nrep; r++) {
-            const int off = c->rh[r].first, nr = s[r].n_particles;
+            const int off = c->rh[r].first;
             c->rh[r].cfirst = (int)list.size();
             for (int b = 0; b < s[r].n_chain_blocks; b++) {
+                // ranges, lengths, overlaps and the constant-set count checked by validate_batch()
                 const int start = s[r].chain[3 * b], cnt = s[r].chain[3 * b + 1], len = s[r].chain[3 * b + 2];
-                if (len < 3 || len > 0xFFFF) unsupported("CHAIN length must be in [3, 65535]");
-                if (cnt < 0 || start < 0 || (long long)start + (long long)cnt * len > nr)
-                    invalid("CHAIN block out of range");
                 const double *cc = s[r].chain_const + 4 * b;
                 const int cid = (int)ccD.size();
-                if (cid > 0x7FFF) unsupported("too many CHAIN constant sets");
                 ccD.push_back(make_double4(cc[0], cc[1], cc[2], cc[3]));
                 for (int k = 0; k < cnt; k++) {
                     const int st0 = off + start + k * len;
                     list.push_back(make_int2(st0, len | (cid << 16)));
-                    for (int j = 0; j < len; j++) {
-                        if (perBead[st0 + j].x != -1) unsupported("a bead belongs to two CHAIN records");
-                        perBead[st0 + j] = make_int2(st0, len | (cid << 16));
-                    }
+                    for (int j = 0; j < len; j++) perBead[st0 + j] = make_int2(st0, len | (cid << 16));
                 }
             }
             c->rh[r].ccount = (int)list.size() - c->rh[r].cfirst;

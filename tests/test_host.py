@@ -165,3 +165,49 @@ def test_no_gamma_stops_like_md(tmp_path):
     import subprocess
     r = subprocess.run([cli, "n"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
     assert r.returncode == 0 and r.stdout == "Error(main): No gamma available!\n"
+
+
+def test_create_refuses_invalid_systems():
+    """sm_create checks its input on the host before any device call, as the reference's
+    Blob::errorChecking does before a run (system.h:438-580): a position outside
+    [0, L] on any axis (system.h:452-469; NaN included), a type outside [0, nTypes), a CHAIN
+    block or BOND record naming a particle past the end (system.h:490-535), and
+    non-positive cutoff, dt, gamma or box are SM_ERR_INVALID; fewer than 3 cells along an
+    axis is SM_ERR_UNSUPPORTED.  A position exactly at L passes the check."""
+    d = sm.Mpd.bilayer(1234, 250, 3.45).system()
+    n = len(d["type"])
+    L = np.asarray(d["box"])
+
+    def with_pos(i, axis, value):
+        p = np.array(d["pos"], copy=True)
+        p[i, axis] = value
+        return dict(d, pos=p)
+
+    bad_type = np.array(d["type"], copy=True)
+    bad_type[7] = d["n_types"]
+    bad_chain = np.array(d["chain"], copy=True)
+    bad_chain[-1, 1] += 1                      # one chain more than the beads hold
+    cases = {
+        "x above L": (with_pos(3, 0, L[0] * (1 + 1e-12)), sm.SM_ERR_INVALID, "out of bounds"),
+        "y below 0": (with_pos(4, 1, -1e-9), sm.SM_ERR_INVALID, "out of bounds"),
+        "z NaN": (with_pos(5, 2, np.nan), sm.SM_ERR_INVALID, "out of bounds"),
+        "type": (dict(d, type=bad_type), sm.SM_ERR_INVALID, "type outside"),
+        "chain": (dict(d, chain=bad_chain), sm.SM_ERR_INVALID, "CHAIN block out of range"),
+        "bond": (dict(d, bond=np.array([[0, n]], np.int32), bond_const=np.array([[0.7, 100.0]])),
+                 sm.SM_ERR_INVALID, "out of range"),
+        "cutoff": (dict(d, cutoff=0.0), sm.SM_ERR_INVALID, "cutoff"),
+        "dt": (dict(d, dt=-0.01), sm.SM_ERR_INVALID, "dt"),
+        "gamma": (dict(d, gamma=0.0), sm.SM_ERR_INVALID, "gamma"),
+        "box": (dict(d, box=np.array([L[0], -L[1], L[2]])), sm.SM_ERR_INVALID, "box"),
+        "cells": (dict(d, box=np.array([L[0], L[1], 5.9]), pos=np.minimum(d["pos"], [L[0], L[1], 5.9])),
+                  sm.SM_ERR_UNSUPPORTED, "fewer than 3 cells"),
+    }
+    for name, (e, code, text) in cases.items():
+        with pytest.raises(sm.SmError) as err:
+            sm.Solver.from_system(e)
+        assert err.value.code == code and text in str(err.value), (name, str(err.value))
+    # at exactly L the check passes (here the device step may then fail for want of a GPU)
+    try:
+        sm.Solver.from_system(with_pos(3, 0, L[0]))
+    except sm.SmError as err:
+        assert err.code != sm.SM_ERR_INVALID, str(err)
