@@ -150,9 +150,10 @@ def fp64_check(d: dict, cores: int) -> dict:
             "checker": "FP64 oracle restatement of MD.cpp, step 0 of the same vesicle"}
 
 
-def bench_batched(steps: int, warmup: int, seq_steps: int) -> dict:
+def bench_batched(steps: int, warmup: int, seq_steps: int, fixed: dict | None = None) -> dict:
     """The 64-replica tension x temperature sweep stepped as one batch, against the same
-    replicas run one after another through the single-system solver."""
+    replicas run one after another through the single-system solver (seq_steps 0: no
+    sequential run).  fixed: knob values to step with instead of the autotuner's choice."""
     import torch
 
     import softmold_amd as sm
@@ -162,7 +163,12 @@ def bench_batched(steps: int, warmup: int, seq_steps: int) -> dict:
     nb = sum(len(d["type"]) for d in systems)
     b = sm.Solver.batch(systems)
     b.init_forces()
-    tuned_ms = b.autotune(32)
+    tuned_ms = None
+    if fixed:
+        for k, v in fixed.items():
+            b.tune(k, v)
+    else:
+        tuned_ms = b.autotune(32)
     knobs = {k: b.tune_get(k) for k in ("pair_chunk", "pair_variant", "graph", "sort_mode")}
     b.run(1, warmup)
     b.sync()
@@ -183,7 +189,7 @@ def bench_batched(steps: int, warmup: int, seq_steps: int) -> dict:
     # stepping (where a step without a box move ends in the fused kick + next-first-half
     # kernel); the same steps and warmup as the batch, summed walls
     seq_wall, seq_work, modes = 0.0, 0, {"graph": 0, "eager": 0}
-    for d in systems:
+    for d in (systems if seq_steps > 0 else []):
         s = sm.Solver.from_system(d)
         s.init_forces()
         s.autotune(32)
@@ -196,14 +202,15 @@ def bench_batched(steps: int, warmup: int, seq_steps: int) -> dict:
         seq_wall += time.perf_counter() - t0
         seq_work += len(d["type"]) * seq_steps
         s.close()
-    seq = seq_work / seq_wall
-    log(f"sequential plain solver: {seq:.4g} particle-steps/s ({seq_steps} steps per replica, "
-        f"autotuned: {modes})")
+    seq = seq_work / seq_wall if seq_steps > 0 else None
+    if seq:
+        log(f"sequential plain solver: {seq:.4g} particle-steps/s ({seq_steps} steps per replica, "
+            f"autotuned: {modes})")
     return {"value": batched, "unit": "particle-steps/s", "replicas": len(systems), "beads": nb,
             "steps": steps, "warmup": warmup, "ms_per_step": 1e3 * wall / steps,
             "sequential_value": seq, "sequential_steps_per_replica": seq_steps,
             "sequential_modes": modes,
-            "speedup_vs_sequential": batched / seq, "autotuned": knobs,
+            "speedup_vs_sequential": batched / seq if seq else None, ("knobs" if fixed else "autotuned"): knobs,
             "autotune_best_ms_per_step": tuned_ms,
             "workload": "64 liposome replicas (seeds 1000..1063), lipid counts log-uniform in "
                         "[2000, 40000] (MT19937 seed 20261202), tension linspace(0,1,8) x "
@@ -282,6 +289,9 @@ def main():
     ap.add_argument("--calib-copy", action="store_true")
     ap.add_argument("--batched-only", action="store_true", help="only the replica sweep (for profiling it)")
     ap.add_argument("--bilayer-only", action="store_true", help="only the 1M-lipid flat bilayer (for profiling it)")
+    ap.add_argument("--batched-knobs", default=None, metavar="CHUNK,SORT_MODE,GRAPH",
+                    help="with --batched-only: step the batch with these knobs, without the autotuner "
+                         "and without the sequential baseline")
     ap.add_argument("--sort-mode", type=int, choices=(0, 1), default=None,
                     help="the plain workload's cell sort: 0 every key, 1 incremental (default: the solver's)")
     args = ap.parse_args()
@@ -292,7 +302,11 @@ def main():
         print(json.dumps({"bilayer": bench_bilayer(args.steps, args.warmup)}), flush=True)
         return
     if args.batched_only:
-        print(json.dumps({"batched": bench_batched(args.batched_steps, 100, args.batched_steps)}), flush=True)
+        fixed, seq_steps = None, args.batched_steps
+        if args.batched_knobs:
+            chunk, sort_mode, graph = (int(v) for v in args.batched_knobs.split(","))
+            fixed, seq_steps = {"pair_chunk": chunk, "sort_mode": sort_mode, "graph": graph}, 0
+        print(json.dumps({"batched": bench_batched(args.batched_steps, 100, seq_steps, fixed)}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the solver has no multi-GPU path "
