@@ -14,6 +14,11 @@
 //    lBond_name.dat, `time\tcos\tla\tlb` of the bends to bend_name.dat, `time\tT` to
 //    temp_name.dat (dataExtraction.h:992-1050, dataCollection.h:677-690) and
 //    `time\tseconds` on stderr;
+//  - at step 0 and every measurement, the extent of the particles along each axis (largest
+//    minus smallest coordinate, from the box and 0) to flicker_name.dat, and the particles'
+//    kinetic energies 0.5 |v|^2 into a histogram of width 1e-4 written as
+//    kEnergyDensity_name.dat at the end (dataExtraction.h:1456-1487,1512-1519,552-567;
+//    MD.cpp builds without FLAT_MEMBRANE and with its unwrapped-position array set);
 //  - temperature ramp by tempStepInterval (MD.cpp:320-325,369-371);
 //  - MC box move every 8 steps when deltaLXY != 0 (MD.cpp:589-721), resizeHist_name.dat
 //    and "Resize acceptance ratio" at exit (MD.cpp:731-748).
@@ -58,6 +63,48 @@ void append_line(const std::string &file, double t, const double *v, int nv)
     f << std::endl;
 }
 
+// flicker_ line: per axis, the largest coordinate (from 0) minus the smallest (from the box
+// length) over all particles (dataExtraction.h:1456-1487)
+void append_flicker(const std::string &file, double t, int n, const double *pos, const double *box)
+{
+    double ext[3];
+    for (int d = 0; d < 3; d++) {
+        double lo = box[d], hi = 0.0;
+        for (int i = 0; i < n; i++) {
+            const double p = pos[3 * (size_t)i + d];
+            if (p < lo) lo = p;
+            if (p > hi) hi = p;
+        }
+        ext[d] = hi - lo;
+    }
+    append_line(file, t, ext, 3);
+}
+
+// kinetic energy histogram of a measurement (dataExtraction.h:1512-1519): bin
+// int(0.5 |v|^2 / 1e-4), grown to the largest bin seen
+constexpr double kKeBin = 0.0001;
+void add_kinetic(std::vector<long> &hist, int n, const double *vel)
+{
+    for (int i = 0; i < n; i++) {
+        const double *v = vel + 3 * (size_t)i;
+        const int bin = (int)(0.5 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / kKeBin);
+        if (bin < 0) continue;
+        if ((size_t)bin >= hist.size()) hist.resize((size_t)bin + 1, 0);
+        hist[bin]++;
+    }
+}
+
+// kEnergyDensity_ at the end of the run (dataExtraction.h:552-567): every bin's lower edge
+// and its share of all counts
+void write_kinetic(const std::string &file, const std::vector<long> &hist)
+{
+    long sum = 0;
+    for (long c : hist) sum += c;
+    std::fstream f(file.c_str(), std::ios::out);
+    for (size_t i = 0; i < hist.size(); i++)
+        f << static_cast<float>(i) * kKeBin << '\t' << static_cast<float>(hist[i]) / static_cast<float>(sum) << '\n';
+}
+
 // xyzFormat::store (xyzFormat.h:112-143): count, "test", then type x y z per particle.
 void append_frame(const std::string &file, int n, const int32_t *type, const double *pos)
 {
@@ -81,7 +128,8 @@ int run(const char *name)
     const std::string fFrames = "frames_" + nm + ".xyz", fPot = "potential_" + nm + ".dat",
                       fKin = "kinetic_" + nm + ".dat", fSize = "size_" + nm + ".dat", fTemp = "temp_" + nm + ".dat",
                       fHist = "resizeHist_" + nm + ".dat", fBond = "lBond_" + nm + ".dat",
-                      fBend = "bend_" + nm + ".dat";
+                      fBend = "bend_" + nm + ".dat", fFlick = "flicker_" + nm + ".dat",
+                      fKe = "kEnergyDensity_" + nm + ".dat";
     int rc = 0;
     sm_ctx *ctx = nullptr;
     try {
@@ -116,6 +164,7 @@ int run(const char *name)
         std::vector<int32_t> type(sys.type, sys.type + n);
         std::vector<double> pos(3 * (size_t)n), vel(3 * (size_t)n);
         double box[3] = {sys.box[0], sys.box[1], sys.box[2]};
+        std::vector<long> kehist;
 
         check(sm_create(&sys, &ctx), "create");
         check(sm_init_forces(ctx), "initial forces");
@@ -130,6 +179,8 @@ int run(const char *name)
             append_line(fBend, t0, bs + 1, 3);
             append_line(fTemp, t0, &temp, 1);
             append_line(fKin, t0, &m[2], 1);
+            append_flicker(fFlick, t0, n, sys.pos, box);
+            add_kinetic(kehist, n, sys.vel);
             append_frame(fFrames, n, type.data(), sys.pos);
         } else {
             check(sm_second_half(ctx), "second half step");
@@ -171,11 +222,14 @@ int run(const char *name)
             // earlier part of the step changes it
             double m[3], b[3], bs[4];
             if (measure) {
-                check(sm_download_state(ctx, nullptr, nullptr, nullptr, b), "box");
+                // positions before this step's MC move, as the measurement sees them
+                check(sm_download_state(ctx, pos.data(), nullptr, nullptr, b), "positions");
                 check(sm_bond_stats(ctx, bs), "bond statistics");   // positions as measured
             }
             check(sm_step_finish(ctx, i, measure ? m : nullptr), "step");
             if (measure) {
+                // velocities after the kick (the MC move leaves them)
+                check(sm_download_state(ctx, nullptr, vel.data(), nullptr, nullptr), "velocities");
                 const time_t last = current;
                 current = time(nullptr);
                 std::cerr << t << '\t' << current - last << std::endl;
@@ -186,6 +240,8 @@ int run(const char *name)
                 append_line(fBend, t, bs + 1, 3);
                 append_line(fTemp, t, &temp, 1);
                 append_line(fKin, t, &m[2], 1);
+                append_flicker(fFlick, t, n, pos.data(), b);
+                add_kinetic(kehist, n, vel.data());
                 current = time(nullptr);
             }
             i++;
@@ -204,6 +260,7 @@ int run(const char *name)
         }
         const double tr = (double)trials, ac = (double)accepted;
         std::cerr << "Resize acceptance ratio: " << ac / tr << std::endl;
+        write_kinetic(fKe, kehist);
     } catch (const Fail &f) {
         rc = f.code;
     }

@@ -7,7 +7,8 @@ binaries print and write.  The fixtures are data only: generator checksums (the 
 systems, the 1M-lipid bilayer and the 64 liposomes of the replica sweep), step-0 FP64
 forces/keys/neighbour counts/energies, short single-thread MD trajectories (also of two
 systems whose two-body tables differ between the (a, b) and (b, a) rows), and the bond
-and bend statistics MD writes for a system with BOND and BEND records.
+and bend statistics, extents and kinetic-energy histogram MD writes at step 0 for a
+system with BOND and BEND records.
 
 usage: python tests/golden/make_golden.py
 """
@@ -188,7 +189,7 @@ def main():
             rec = {"steps": c["steps"],
                    "input_mpd": open(os.path.join(d, "input.mpd")).read(),
                    "final_mpd": open(os.path.join(d, tname + ".mpd")).read()}
-            for dat in ("potential", "kinetic", "size"):
+            for dat in ("potential", "kinetic", "size", "flicker"):
                 rec[dat] = read_dat(os.path.join(d, f"{dat}_{tname}.dat"))
             hist = os.path.join(d, f"resizeHist_{tname}.dat")
             if os.path.exists(hist):
@@ -205,6 +206,12 @@ def main():
         run([os.path.join(REF, "MD"), "BS"], d)
         rec["lBond"] = read_dat(os.path.join(d, "lBond_BS.dat"))
         rec["bend"] = read_dat(os.path.join(d, "bend_BS.dat"))
+        rec["flicker"] = read_dat(os.path.join(d, "flicker_BS.dat"))
+        # the kinetic-energy histogram of the step-0 velocities (dataExtraction.h:552-567,
+        # 1512-1519): its line count and its nonzero lines
+        ke = read_dat(os.path.join(d, "kEnergyDensity_BS.dat"))
+        rec["kEnergyDensity_lines"] = np.array([len(ke)])
+        rec["kEnergyDensity_nonzero"] = ke[ke[:, 1] != 0]
         traj["BS"] = rec
 
         with open(os.path.join(HERE, "gen_sha256.json"), "w") as f:

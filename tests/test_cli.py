@@ -178,3 +178,40 @@ def test_cli_restart_and_mc(tmp_path):
     kin1 = _table(tmp_path / "kinetic_one.dat")
     np.testing.assert_allclose(kin1[:5], kin[:5], rtol=1e-12)
     np.testing.assert_allclose(kin1[5:, 1], kin[5:, 1], rtol=0.1)
+
+
+@pytest.mark.gpu
+def test_cli_extents_and_kinetic_histogram_match_reference(golden_traj, tmp_path):
+    """flicker_ (the particles' extent along each axis at step 0 and every measurement) and
+    kEnergyDensity_ (the histogram of 0.5 |v|^2 in bins of 1e-4 over those measurements,
+    dataExtraction.h:552-567,1456-1487,1512-1519) against the reference MD's own files: on
+    the BOND/BEND system run for step 0 only, both files equal the reference's (the
+    step-0 velocities straddle the bin edge at 4.5, so the split between two bins checks
+    the FP64 bin arithmetic); on AB400 over 40 steps, the step-0 extents equal the
+    reference's and the later lines keep its format (their values follow the noise)."""
+    if not gpu_available():
+        pytest.skip("no GPU")
+    (tmp_path / "BS.mpd").write_bytes(golden_traj["BS/input_mpd"].tobytes())
+    print("case: md_hip BS step 0", flush=True)
+    r = _run(["BS"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    np.testing.assert_array_equal(_table(tmp_path / "flicker_BS.dat"), golden_traj["BS/flicker"])
+    ke = _table(tmp_path / "kEnergyDensity_BS.dat")
+    assert len(ke) == int(golden_traj["BS/kEnergyDensity_lines"][0])
+    np.testing.assert_array_equal(ke[ke[:, 1] != 0], golden_traj["BS/kEnergyDensity_nonzero"])
+    np.testing.assert_allclose(ke[:, 0], np.arange(len(ke)) * 1e-4, rtol=1e-5, atol=1e-9)
+
+    name = "AB400t"
+    (tmp_path / f"{name}.mpd").write_bytes(golden_traj[name + "/input_mpd"].tobytes())
+    print("case: md_hip AB400t 40 steps", flush=True)
+    r = _run([name], tmp_path)
+    assert r.returncode == 0, r.stderr
+    fl = _table(tmp_path / f"flicker_{name}.dat")
+    ref = golden_traj[name + "/flicker"]
+    assert fl.shape == ref.shape
+    np.testing.assert_array_equal(fl[0], ref[0])
+    np.testing.assert_array_equal(fl[:, 0], _table(tmp_path / f"potential_{name}.dat")[:, 0])
+    size = _table(tmp_path / f"size_{name}.dat")
+    assert np.all(fl[:, 1:] > 0) and np.all(fl[:, 1:] <= size[:, 1:] * (1 + 1e-6))
+    ke = _table(tmp_path / f"kEnergyDensity_{name}.dat")
+    assert ke[:, 1].sum() == pytest.approx(1.0, abs=1e-4)
