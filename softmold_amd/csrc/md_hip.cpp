@@ -81,14 +81,18 @@ void append_flicker(const std::string &file, double t, int n, const double *pos,
 }
 
 // kinetic energy histogram of a measurement (dataExtraction.h:1512-1519): bin
-// int(0.5 |v|^2 / 1e-4), grown to the largest bin seen
+// int(0.5 |v|^2 / 1e-4), grown to the largest bin seen.  A value that is not finite or
+// lies past bin 2^28 (0.5 |v|^2 > 26843, thousands of times the thermal energy) is not
+// counted: the reference would convert it to int undefined or grow its table to gigabytes.
 constexpr double kKeBin = 0.0001;
+constexpr double kKeMaxBin = 268435456.0;
 void add_kinetic(std::vector<long> &hist, int n, const double *vel)
 {
     for (int i = 0; i < n; i++) {
         const double *v = vel + 3 * (size_t)i;
-        const int bin = (int)(0.5 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / kKeBin);
-        if (bin < 0) continue;
+        const double e = 0.5 * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / kKeBin;
+        if (!(e >= 0.0 && e < kKeMaxBin)) continue;
+        const int bin = (int)e;
         if ((size_t)bin >= hist.size()) hist.resize((size_t)bin + 1, 0);
         hist[bin]++;
     }
