@@ -342,6 +342,9 @@ __global__ __launch_bounds__(kBlock) void k_bonded_kick(int n, const Bead *__res
 // k_bonded_kick followed by k_first.  The drifted positions go to beads_out, since other
 // beads' bonded forces still read this step's positions; the step counter is taken as an
 // argument (no thread reads the device counter, which thread 0 advances to step + 1).
+// ONE: a single-replica context, whose replica constants are read from reps[0] without
+// waiting for the bead's replica index
+template <bool ONE>
 __global__ __launch_bounds__(kBlock) void k_kick_first(int n, const Bead *__restrict__ beads,
                                                        Bead *__restrict__ beads_out,
                                                        const int2 *__restrict__ chain,
@@ -361,7 +364,7 @@ __global__ __launch_bounds__(kBlock) void k_kick_first(int n, const Bead *__rest
     uint32_t k = 0u;
     if (i < n) {
         Bead b = beads[i];
-        const DevRep &R = reps[b.rep];
+        const DevRep &R = reps[ONE ? 0 : b.rep];
         const double3 fc = bonded_force(i, b, beads, chain, cconst, gen, R.L[0], R.L[1], R.L[2]);
         float4 v = vel[i];
         const float4 a = total_accel(i, R, fc, fpair[i], v, step, TAG_LANGEVIN);
@@ -783,8 +786,15 @@ void launch_kick_first(hipStream_t st, const Params &P, const Bead *beads, Bead 
                        const DevRep *reps, DevGlobal *glob, long long step, uint32_t *key, uint32_t *idx,
                        uint32_t *ghist, int passes, const MoveBook &book)
 {
-    hipLaunchKernelGGL(k_kick_first, dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, beads, beads_out, chain, cconst, gen,
-                       fpair, vel, acc, reps, glob, (float)(P.dt * 0.5), P.dt, step, key, idx, ghist, passes, book);
+    const float halfdt = (float)(P.dt * 0.5);
+    if (P.nrep == 1)
+        hipLaunchKernelGGL((k_kick_first<true>), dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, beads, beads_out,
+                           chain, cconst, gen, fpair, vel, acc, reps, glob, halfdt, P.dt, step, key, idx, ghist,
+                           passes, book);
+    else
+        hipLaunchKernelGGL((k_kick_first<false>), dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, beads, beads_out,
+                           chain, cconst, gen, fpair, vel, acc, reps, glob, halfdt, P.dt, step, key, idx, ghist,
+                           passes, book);
 }
 
 void launch_second(hipStream_t st, int n, const Bead *beads, const float4 *acc, float4 *vel, float halfdt)

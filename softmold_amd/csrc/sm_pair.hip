@@ -342,8 +342,10 @@ struct Out {
 };
 
 // ASYM: the two-body table's (a, b) and (b, a) rows may differ, and each pair takes the row
-// of the particle the reference's traversal makes first (home_first).
-template <int MODE, bool ASYM>
+// of the particle the reference's traversal makes first (home_first).  ONE: a
+// single-replica context, whose box and grid are read from reps[0] once rather than per
+// cell after the home bead's replica index.
+template <int MODE, bool ASYM, bool ONE = false>
 __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(4))) void k_cells(
     int n, int nT, int chunk, int nblocks, double cut2, float cut2pre, const Bead *__restrict__ sb,
     const uint32_t *__restrict__ skey, const uint32_t *__restrict__ sidx, const int *__restrict__ cstart,
@@ -428,7 +430,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
             const uint32_t kg = skey[s0];
             const int e0 = cend[kg];
             // the home cell's replica: box, grid and key offset (wave-uniform)
-            const DevRep &R = reps[sb[s0].rep];
+            const DevRep &R = reps[ONE ? 0 : sb[s0].rep];
             const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
             const double Lx = R.L[0], Ly = R.L[1], Lz = R.L[2];
             const double csx = R.cs[0], csy = R.cs[1], csz = R.cs[2];
@@ -791,7 +793,7 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     }
 }
 
-template <int MODE, bool ASYM = false>
+template <int MODE, bool ASYM = false, bool ONE = false>
 void launch_cells(hipStream_t st, const Params &P, int chunk, const Bead *sbeads, const uint32_t *skey,
                   const uint32_t *sidx, const int *cstart, const int *cend, const DevRep *reps,
                   const float *fconst, const double *uconst, const Out &out)
@@ -806,7 +808,7 @@ void launch_cells(hipStream_t st, const Params &P, int chunk, const Bead *sbeads
     // the two-phase kernel's FP32 screening radius (the culled modes take theirs per
     // candidate type from the constant tables, in the kernel)
     const float cut2pre = (float)(cut2 + slack);
-    hipLaunchKernelGGL((k_cells<MODE, ASYM>), dim3(nb), dim3(kWaves * kWave), 0, st, P.n, P.nT, chunk, nb, cut2, cut2pre,
+    hipLaunchKernelGGL((k_cells<MODE, ASYM, ONE>), dim3(nb), dim3(kWaves * kWave), 0, st, P.n, P.nT, chunk, nb, cut2, cut2pre,
                        sbeads, skey, sidx, cstart, cend, reps, fconst, uconst, out);
 }
 
@@ -827,8 +829,12 @@ void launch_pair_force(hipStream_t st, const Params &P, int chunk, int variant, 
         launch_cells<M_FORCE, true>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, fconst, nullptr, o);
     else if (variant == 1)
         launch_cells<M_FORCE>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, fconst, nullptr, o);
+    else if (P.asym && P.nrep == 1)
+        launch_cells<M_FORCE1, true, true>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, fconst, nullptr, o);
     else if (P.asym)
         launch_cells<M_FORCE1, true>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, fconst, nullptr, o);
+    else if (P.nrep == 1)
+        launch_cells<M_FORCE1, false, true>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, fconst, nullptr, o);
     else
         launch_cells<M_FORCE1>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, fconst, nullptr, o);
 }
@@ -838,8 +844,12 @@ void launch_pair_force_du(hipStream_t st, const Params &P, int chunk, const Bead
                           const float *fconst, const double *uconst, float4 *fpair, double *energy)
 {
     Out o{fpair, energy, nullptr, nullptr, nullptr, 0};
-    if (P.asym)
+    if (P.asym && P.nrep == 1)
+        launch_cells<M_FDU, true, true>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, fconst, uconst, o);
+    else if (P.asym)
         launch_cells<M_FDU, true>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, fconst, uconst, o);
+    else if (P.nrep == 1)
+        launch_cells<M_FDU, false, true>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, fconst, uconst, o);
     else
         launch_cells<M_FDU>(st, P, chunk, sbeads, skey, sidx, cstart, cend, reps, fconst, uconst, o);
 }
