@@ -395,34 +395,75 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
     // beyond r_m such pairs have zero force and potential, so the candidates culled by it
     // contribute exactly zero.
     __shared__ float sReach2[kMaxTypes];
-    if (kF) {
-        const float4 *t8 = reinterpret_cast<const float4 *>(tab);
-        for (int t = threadIdx.x; t < 2 * nT * nT; t += blockDim.x) sC8[t] = t8[t];
-    }
-    if (!kF || kFDU) {
-        for (int t = threadIdx.x; t < 6 * nT * nT; t += blockDim.x) sCd[t] = uconst[t];
-    }
-    if (kCull && (int)threadIdx.x < nT) {
-        const float rf = tab[8 * nT * nT + threadIdx.x], re = tab[8 * nT * nT + kMaxTypes + threadIdx.x];
-        sReach2[threadIdx.x] = kFDU ? fmaxf(rf, re) : (kE ? re : rf);
-    }
-    __syncthreads();
-
     const int lane = threadIdx.x & (kWave - 1);
-    const unsigned long long lanes_lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     // wave index, wave-uniform (the compiler then keeps the chunk's cell setup scalar)
     const int wave = kWaves == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
     const long long a0 = ((long long)xcd_block<kE ? 64 : 0>(blockIdx.x, nblocks) * kWaves + wave) * chunk;
     const int a = (int)min(a0, (long long)n);
     const int aend = min(a + chunk, n);
+    // the first window's keys (this slot's and the one before), loaded before the constant
+    // tables are staged so that the two memory trips overlap
+    uint32_t kc = 0u, kp = 0u;
+    if (a + lane < aend) {
+        kc = skey[a + lane];
+        if (a + lane > 0) kp = skey[a + lane - 1];
+    }
+    // the constant tables: every load first (indices clamped in bounds, so no branch holds
+    // one back), then the LDS stores, so the staging costs one memory trip
+    constexpr int kThreads = kWaves * kWave;
+    constexpr int kC8Per = (2 * kMaxTypes * kMaxTypes + kThreads - 1) / kThreads;
+    constexpr int kCdPer = (6 * kMaxTypes * kMaxTypes + kThreads - 1) / kThreads;
+    float4 c8[kF ? kC8Per : 1];
+    double cd[(!kF || kFDU) ? kCdPer : 1];
+    float rf = 0.f, re = 0.f;
+    if (kF) {
+        const float4 *t8 = reinterpret_cast<const float4 *>(tab);
+#pragma unroll
+        for (int k = 0; k < kC8Per; k++) {
+            const int t = threadIdx.x + k * kThreads;
+            c8[k] = t8[min(t, 2 * nT * nT - 1)];
+        }
+    }
+    if (!kF || kFDU) {
+#pragma unroll
+        for (int k = 0; k < kCdPer; k++) {
+            const int t = threadIdx.x + k * kThreads;
+            cd[k] = uconst[min(t, 6 * nT * nT - 1)];
+        }
+    }
+    const int treach = min((int)threadIdx.x, nT - 1);   // threads past nT repeat the last type
+    if (kCull) {
+        rf = tab[8 * nT * nT + treach];
+        re = tab[8 * nT * nT + kMaxTypes + treach];
+    }
+    // stored whole (entries past the table hold copies of its last one and are never read),
+    // so that no branch sinks a load down to its store
+    static_assert(kC8Per * kThreads <= 2 * kMaxTypes * kMaxTypes && kCdPer * kThreads <= 6 * kMaxTypes * kMaxTypes,
+                  "the tables are stored whole");
+    if (kF) {
+#pragma unroll
+        for (int k = 0; k < kC8Per; k++) sC8[threadIdx.x + k * kThreads] = c8[k];
+    }
+    if (!kF || kFDU) {
+#pragma unroll
+        for (int k = 0; k < kCdPer; k++) sCd[threadIdx.x + k * kThreads] = cd[k];
+    }
+    // every thread stores (those past nT the last type's value again, the same bits)
+    if (kCull) sReach2[treach] = kFDU ? fmaxf(rf, re) : (kE ? re : rf);
+    __syncthreads();
+
+    const unsigned long long lanes_lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     constexpr uint32_t fwd = forward_mask();
     Cand *Cw = sCand[wave];
     float4 *Rw = sRel[k2P ? wave : 0];
 
     for (int base = a; base < aend; base += kWave) {
         const int s = base + lane;
-        bool st = false;
-        if (s < aend) st = (s == 0) || (skey[s - 1] != skey[s]);
+        if (base != a && s < aend) {
+            kc = skey[s];
+            kp = skey[s - 1];
+        }
+        const bool st = s < aend && (s == 0 || kp != kc);
         unsigned long long starts = __ballot(st);
         while (starts) {
             const int s0 = base + __ffsll((long long)starts) - 1;
