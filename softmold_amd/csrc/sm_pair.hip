@@ -468,7 +468,8 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
         while (starts) {
             const int s0 = base + __ffsll((long long)starts) - 1;
             starts &= starts - 1;
-            const uint32_t kg = skey[s0];
+            // the home cell's key from the lane that loaded it
+            const uint32_t kg = (uint32_t)__builtin_amdgcn_readlane((int)kc, s0 - base);
             const int e0 = cend[kg];
             // the home cell's replica: box, grid and key offset (wave-uniform)
             const DevRep &R = reps[ONE ? 0 : sb[s0].rep];
