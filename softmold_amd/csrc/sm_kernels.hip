@@ -163,10 +163,11 @@ __global__ __launch_bounds__(kBlock) void k_reorder(int n, const uint32_t *__res
     const int s = blockIdx.x * kBlock + threadIdx.x;
     for (size_t w = (size_t)s; w < clear_words; w += (size_t)gridDim.x * kBlock) clear[w] = 0u;
     if (s >= n) return;
-    const uint32_t k = skey[s];
+    // every load issued at once (the neighbours' indices clamped in bounds)
+    const uint32_t k = skey[s], kprev = skey[s > 0 ? s - 1 : 0], knext = skey[s < n - 1 ? s + 1 : s];
     sbeads[s] = beads[sidx[s]];
-    if (s == 0 || skey[s - 1] != k) cstart[k] = s;
-    if (s == n - 1 || skey[s + 1] != k) cend[k] = s + 1;
+    if (s == 0 || kprev != k) cstart[k] = s;
+    if (s == n - 1 || knext != k) cend[k] = s + 1;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -221,21 +222,23 @@ __device__ inline double3 bend_f(double3 da, double3 db, double c0, double kt, i
 }
 
 // CHAIN and generic bonded force on bead i (FP64)
-__device__ inline double3 bonded_force(int i, const Bead &p0, const Bead *__restrict__ beads,
+__device__ inline double3 bonded_force(int i, int n, const Bead &p0, const Bead *__restrict__ beads,
                                        const int2 *__restrict__ chain, const double4 *__restrict__ cconst,
                                        const GenTables &gen, double L0, double L1, double L2)
 {
     double3 fc = make_double3(0.0, 0.0, 0.0);
+    // the four beads around i, loaded with the chain record rather than after it (indices
+    // clamped in bounds; a bead outside this one's chain is loaded and not used)
+    const Bead pm2 = beads[max(i - 2, 0)], pm1 = beads[max(i - 1, 0)];
+    const Bead pp1 = beads[min(i + 1, n - 1)], pp2 = beads[min(i + 2, n - 1)];
     const int2 ci = chain[i];
-    if (ci.x >= 0) {
-        const int start = ci.x, len = ci.y & 0xFFFF, cid = ci.y >> 16;
+    // evaluated for every bead and kept only for a chain bead (no branch for the compiler to
+    // sink the neighbour loads into); a bead outside any chain reads constant set 0
+    {
+        const bool inchain = ci.x >= 0;
+        const int start = inchain ? ci.x : i, len = inchain ? (ci.y & 0xFFFF) : 1, cid = inchain ? (ci.y >> 16) : 0;
         const double4 K = cconst[cid];
         const int j = i - start;
-        Bead pm1, pm2, pp1, pp2;
-        if (j >= 1) pm1 = beads[i - 1];
-        if (j >= 2) pm2 = beads[i - 2];
-        if (j <= len - 2) pp1 = beads[i + 1];
-        if (j <= len - 3) pp2 = beads[i + 2];
         // the chain differences around this bead, each with its length and inverse length
         // computed once (the bonds and bends below share them)
         const double3 z3 = make_double3(0.0, 0.0, 0.0);
@@ -323,7 +326,7 @@ __global__ __launch_bounds__(kBlock) void k_bonded_kick(int n, const Bead *__res
     if (i >= n) return;
     const Bead p0 = beads[i];
     const DevRep &R = reps[p0.rep];
-    const double3 fc = bonded_force(i, p0, beads, chain, cconst, gen, R.L[0], R.L[1], R.L[2]);
+    const double3 fc = bonded_force(i, n, p0, beads, chain, cconst, gen, R.L[0], R.L[1], R.L[2]);
     if (mode == KICK_NONE) {
         fchain_out[i] = make_float4((float)fc.x, (float)fc.y, (float)fc.z, 0.f);
         return;
@@ -365,7 +368,7 @@ __global__ __launch_bounds__(kBlock) void k_kick_first(int n, const Bead *__rest
     if (i < n) {
         Bead b = beads[i];
         const DevRep &R = reps[ONE ? 0 : b.rep];
-        const double3 fc = bonded_force(i, b, beads, chain, cconst, gen, R.L[0], R.L[1], R.L[2]);
+        const double3 fc = bonded_force(i, n, b, beads, chain, cconst, gen, R.L[0], R.L[1], R.L[2]);
         float4 v = vel[i];
         const float4 a = total_accel(i, R, fc, fpair[i], v, step, TAG_LANGEVIN);
         acc[i] = a;
