@@ -470,7 +470,6 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
             starts &= starts - 1;
             // the home cell's key from the lane that loaded it
             const uint32_t kg = (uint32_t)__builtin_amdgcn_readlane((int)kc, s0 - base);
-            const int e0 = cend[kg];
             // the home cell's replica: box, grid and key offset (wave-uniform)
             const DevRep &R = reps[ONE ? 0 : sb[s0].rep];
             const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
@@ -515,19 +514,22 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                 Y0 = Y0 < 0 ? Y0 + ny : (Y0 >= ny ? Y0 - ny : Y0);
                 Z0 = Z0 < 0 ? Z0 + nz : (Z0 >= nz ? Z0 - nz : Z0);
                 const uint32_t kk = kbase + (uint32_t)(X0 + Y0 * nx + Z0 * nx * ny);
+                // both ends in one trip, then the run's key (index clamped in bounds): no
+                // branch for the compiler to sink a load into
                 const int c0 = cstart[kk];
                 const int c1 = cend[kk];
-                if ((unsigned)c0 < (unsigned)n && skey[c0] == kk) {
-                    nst = c0;
-                    ncnt = c1 - c0;
-                }
+                const bool ok = (unsigned)c0 < (unsigned)n && skey[min(max(c0, 0), n - 1)] == kk;
+                nst = ok ? c0 : 0;
+                ncnt = ok ? c1 - c0 : 0;
             }
             // 2. layout of the candidates
             const int incl = wave_incl_scan(ncnt, lane);
             const int excl = incl - ncnt;
             const int total = __builtin_amdgcn_readlane(incl, 26);
             const int selfOff = __builtin_amdgcn_readlane(excl, 13);
-            const int nh = e0 - s0;
+            // home particles: the home cell's own lookup (stencil cell 13 holds it, and the
+            // cell table entry of the cell being sorted into is current)
+            const int nh = __builtin_amdgcn_readlane(ncnt, 13);
             Shift *ShW = sShift[wave];
             if (lane < 27) {
                 ShW[lane] = shl;
