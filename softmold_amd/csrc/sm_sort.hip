@@ -63,7 +63,17 @@ __global__ __launch_bounds__(kThreads) void k_onesweep(int n, const int *__restr
     const int wave = t / kWave;
     const bool many = ndev != nullptr;
     if (many) n = *ndev;
-    if (t == 0) s_tile = atomicAdd(tilectr, 1u);
+    // the tile id reaches LDS only after the histogram loads below are issued, so the
+    // atomic's round trip overlaps theirs
+    uint32_t first_tile = 0u;
+    {
+        // the counter's address held in vector registers: for a uniform address the compiler
+        // would aggregate the atomic over the wave and wait for its result at once
+        typedef __attribute__((address_space(1))) uint32_t global_u32;
+        global_u32 *ctr = (global_u32 *)tilectr;
+        asm volatile("" : "+v"(ctr));
+        if (t == 0) first_tile = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     // digit base: keys of smaller digits over all keys, from every histogram copy; a scan
     // inside each wave by shuffles, then the waves' totals (one barrier instead of the 16 of
     // an LDS scan: 37.2 against 38.3 us per 240k-key sort)
@@ -78,6 +88,7 @@ __global__ __launch_bounds__(kThreads) void k_onesweep(int n, const int *__restr
         if (lane >= o) x += y;
     }
     if (lane == kWave - 1) scan[wave] = x;
+    if (t == 0) s_tile = first_tile;
     __syncthreads();
     uint32_t wpre = 0;
     for (int w = 0; w < wave; w++) wpre += scan[w];
