@@ -543,14 +543,13 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                 const int hi = lane / tpp;
                 const int sub = lane - hi * tpp;
                 const bool act = hi < H;
-                Bead hb;
-                hb.x = hb.y = hb.z = 0.0;
-                hb.type = 0;
-                if (act) hb = sb[s0 + h0 + hi];
+                // the home particle and its index: loaded for every lane (an idle lane takes the
+                // cell's first) and first used after the pass below has issued its loads, so
+                // the trips overlap; every use is under `act`
                 const int self = selfOff + h0 + hi;
                 const int hslot = s0 + h0 + hi;
-                const uint32_t me = act ? sidx[s0 + h0 + hi] : 0u;
-                const float hfx = (float)(hb.x - ox0), hfy = (float)(hb.y - oy0), hfz = (float)(hb.z - oz0);
+                const Bead hb0 = sb[act ? hslot : s0];
+                const uint32_t me = sidx[act ? hslot : s0];
                 double fx = 0.0, fy = 0.0, fz = 0.0;
                 double accu = 0.0;
                 int cnt = 0;
@@ -644,6 +643,13 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         }
                     }
                     wave_lds_sync();
+                    // the home particle for this pass: the empty asm keeps every use of it
+                    // (its quarter, its table rows) here, after the pass's loads, where the
+                    // compiler would otherwise hoist them ahead of the pass loop and wait there
+                    Bead hb = hb0;
+                    asm volatile("" : "+v"(hb.x), "+v"(hb.y), "+v"(hb.z), "+v"(hb.type));
+                    // home position relative to the cell centre (its quarter; the two-phase screen)
+                    const float hfx = (float)(hb.x - ox0), hfy = (float)(hb.y - oy0), hfz = (float)(hb.z - oz0);
                     if (kFDU && act) {
                         // the force of every candidate and, for each pair this side owns
                         // (forward half shell; the higher slot within the home cell), the
