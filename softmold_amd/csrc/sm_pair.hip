@@ -657,8 +657,13 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                         const int quad = (hfx >= 0.f ? 1 : 0) + (hfy >= 0.f ? 2 : 0);
                         const uint8_t *lst = mapW + quad * kMaxC;
                         const int nl = quad == 0 ? nq[0] : (quad == 1 ? nq[1] : (quad == 2 ? nq[2] : nq[3]));
+                        // the next entry is read one iteration ahead (entries past nl fall in
+                        // the next list or the padding and are never used)
+                        int inext = lst[sub];
                         for (int jj = sub; jj < nl; jj += tpp) {
-                            const Cand q = load_cand(Cw + lst[jj]);
+                            const int i0 = inext;
+                            inext = lst[jj + tpp];
+                            const Cand q = load_cand(Cw + i0);
                             const int tq = q.t & 0xff, c = q.t >> 8;
                             const double dxd = hb.x - q.x, dyd = hb.y - q.y, dzd = hb.z - q.z;
                             const double r2 = sq3(dxd, dyd, dzd);
@@ -766,8 +771,10 @@ __global__ __launch_bounds__(kWaves * kWave) __attribute__((amdgpu_waves_per_eu(
                             if (MODE == M_U) return r2 < cut2 ? pair_u_any(r2, C) : 0.0;
                             return du_pair(dx, dy, dz, r2, C, cut2, s2x, s2y, s2z, dseries);
                         };
+                        int inext = lst[sub];   // read one iteration ahead, as above
                         for (int jj = sub; jj < nl; jj += tpp) {
-                            const int i0 = lst[jj];
+                            const int i0 = inext;
+                            inext = lst[jj + tpp];
                             accu += term(Cw[i0], cb + i0);
                         }
                     } else if (!kF && act) {
