@@ -292,8 +292,9 @@ def main():
     ap.add_argument("--batched-knobs", default=None, metavar="CHUNK,SORT_MODE,GRAPH",
                     help="with --batched-only: step the batch with these knobs, without the autotuner "
                          "and without the sequential baseline")
-    ap.add_argument("--sort-mode", type=int, choices=(0, 1), default=None,
-                    help="the plain workload's cell sort: 0 every key, 1 incremental (default: the solver's)")
+    ap.add_argument("--sort-mode", type=int, choices=(0, 1, 2), default=None,
+                    help="the plain workload's cell sort: 0 every key, 1 incremental, 2 every key in a window "
+                         "(default: the solver's)")
     args = ap.parse_args()
     if args.calib_copy:
         calib_copy()
@@ -347,6 +348,8 @@ def main():
     s.run(args.warmup + args.steps, args.steps)
     prof = s.profile_read()
     s.profile(False)
+    sort_mode = s.tune_get("sort_mode")
+    window_path = s.tune_get("sort_window_path") == 1
     pair_ms = prof["pair_force"][0]
     for k, (avg, cnt) in prof.items():
         if cnt:
@@ -358,10 +361,12 @@ def main():
     tflops = flops / (pair_ms * 1e-3) / 1e12
     log(f"N={n} occupied cells={occ} candidates/bead={cand:.1f} neighbours/bead={accepted:.1f}")
     # the whole step against HBM: the per-bead algorithmic bytes of an unfused FP32 step
-    # (integrate + key 76, radix sort 16 per 8-bit pass, reorder 36, pair 28, bonded 40,
-    # scatter 16, second half + Langevin 36) and 8 B per occupied cell
+    # (integrate + key 76, radix sort 16 per pass, reorder 36, pair 28, bonded 40,
+    # scatter 16, second half + Langevin 36) and 8 B per occupied cell; the sort takes
+    # ceil(bits / 8) passes over the cell indices, or two over the keys of a window of at
+    # most 2^18 cells
     bits = max(1, int(np.ceil(np.log2(float(np.prod(nc))))))
-    passes = (bits + 7) // 8
+    passes = 2 if window_path else (bits + 7) // 8
     step_bytes = (232.0 + 16.0 * passes) * n + 8.0 * occ
     step_gbs = step_bytes / (ms_per_step * 1e-3) / 1e9
 
@@ -395,7 +400,7 @@ def main():
                              "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": algo_bytes}},
         "step_roofline": {"bound": "hbm", "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": step_gbs / HBM_PEAK_GBS, "algorithmic_bytes_per_step": step_bytes,
-                          "sort_passes": passes},
+                          "sort_passes": passes, "sort_mode": sort_mode},
     }
     s.close()
     if not args.no_batched:

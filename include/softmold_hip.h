@@ -181,15 +181,20 @@ int sm_potential_delta(sm_ctx *ctx, const double *scale, double *du_pair, double
  * "graph" = 1 to replay sm_run's aligned 8-step blocks as one captured hipGraph;
  * "sort_mode" = 1 to re-sort the cell keys incrementally -- the beads whose cell changed
  * are sorted and merged into the previous order (the default from a million beads up) --
- * or 0 to sort every key each build.  Both give the same order, so neither changes a
- * result.  "mc_fused" = 1 (default) to take a box-move step's pair dU in the traversal of
+ * 0 to sort every key each build, or 2 (single-replica contexts of at most 524288 beads,
+ * their default) to sort every key within a window of cells around the previous build's
+ * occupied cells: two 9-bit passes when every key lies in it, else the cell indices' 8-bit
+ * passes, decided on the device.  All give the same order, so none changes a result.
+ * "mc_fused" = 1 (default) to take a box-move step's pair dU in the traversal of
  * that step's pair force, 0 in a gather of its own after the kick (sm_potential_delta
- * follows the same knob); the two sum the same FP64 terms in different orders. */
+ * follows the same knob); the two sum the same FP64 terms in different orders.
+ * sm_tune_get also reads "sort_window_path": 1 if the last build's sort took the window
+ * path (it synchronises the context's stream). */
 int sm_tune(sm_ctx *ctx, const char *knob, int32_t value);
 int sm_tune_get(sm_ctx *ctx, const char *knob, int32_t *value);
 /* Autotuner (no reference counterpart): times `steps` steps of this context's own work
  * for each pair_chunk in {2,4,8,16,32,64} with the one-phase kernel and the graph on, then
- * the best with the other sort_mode, then the best without the graph, from one saved
+ * the best with each other sort_mode, then the best without the graph, from one saved
  * state; keeps the fastest setting, restores
  * the state, and returns the best ms per step.  These settings leave every result
  * bit-identical, so tuning never changes a trajectory. */

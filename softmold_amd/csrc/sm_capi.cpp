@@ -155,7 +155,8 @@ struct sm_ctx {
     // MI355X: 38.4 against 48.9 us per step at 240k beads (the movers' passes and the
     // merge are latency-bound launches), 126 against 115 us at 3M, 156 against 118 us for
     // the 64-replica sweep of 2.8M; contexts of a million beads or more start in mode 1,
-    // and the autotuner tries both.
+    // single replicas that the window sort fits start in mode 2, and the autotuner tries
+    // every mode the context allows.
     int sort_mode = 0;
     uint32_t *kidx[2] = {nullptr, nullptr};
     uint32_t *sk[2] = {nullptr, nullptr}, *sv[2] = {nullptr, nullptr};
@@ -164,6 +165,14 @@ struct sm_ctx {
     int *mcount = nullptr;
     int kcur = 0, scur = 0;
     bool order_valid = false;
+    // Window sort (sort_mode 2, when win_ok: a single replica of at most 256 sort tiles; it
+    // runs while the keys have 17 to 24 bits, the every-key sort otherwise): wdesc + 8p is
+    // the window of the sort with scratch parity p, which the previous sort's reorder
+    // derived on the device; (k2, v2) is the fallback's third buffer pair and then holds the
+    // sorted cell indices; win_last: the last sort was a window sort.
+    bool win_ok = false, win_last = false;
+    uint32_t *k2 = nullptr, *v2 = nullptr, *wdesc = nullptr;
+    bool win_now() const { return sort_mode == 2 && win_ok && sort_bits >= 17 && sort_bits <= 24; }
     int *cstart = nullptr, *cend = nullptr;
     long long capacity = 0;   // cells in the dense table, all replicas
     int sort_bits = 0;
@@ -214,14 +223,25 @@ struct sm_ctx {
     double prof_ms[K_NCLASS] = {};
     long long prof_cnt[K_NCLASS] = {};
 
-    const uint32_t *skey() const { return sort_mode ? sk[scur] : (sorted_sel ? k1 : k0); }
-    const uint32_t *sidx() const { return sort_mode ? sv[scur] : (sorted_sel ? v1 : v0); }
+    const uint32_t *skey() const { return sort_mode == 1 ? sk[scur] : (win_last ? k2 : (sorted_sel ? k1 : k0)); }
+    const uint32_t *sidx() const { return sort_mode == 1 ? sv[scur] : (win_last ? v0 : (sorted_sel ? v1 : v0)); }
     // the key kernel's outputs and bookkeeping for the next sort
-    uint32_t *key_out() const { return sort_mode ? kidx[kcur] : k0; }
-    uint32_t *idx_out() const { return sort_mode ? nullptr : v0; }
+    uint32_t *key_out() const { return sort_mode == 1 ? kidx[kcur] : k0; }
+    uint32_t *idx_out() const { return sort_mode == 1 ? nullptr : v0; }
+    WinBook win_keys() const
+    {
+        if (!win_now()) return WinBook{nullptr, nullptr, nullptr, nullptr};
+        return win_book(hist[sort_parity], n, wdesc + 8 * sort_parity);
+    }
+    // a key kernel ran without its sort: zero what it added to the sort's scratch
+    void restart_hist(uint32_t *gh)
+    {
+        launch_fill_u32(st, gh, sort_hist_words(), 0u);
+        if (win_now()) launch_fill_u32(st, win_scratch(gh, n), kWinKeyWords, 0u);
+    }
     MoveBook move_book() const
     {
-        if (!sort_mode) return MoveBook{nullptr, nullptr};
+        if (sort_mode != 1) return MoveBook{nullptr, nullptr};
         return MoveBook{order_valid ? kidx[1 - kcur] : nullptr, cnt_b};
     }
     double *meas_dev() const { return dres; }
@@ -235,6 +255,7 @@ struct sm_ctx {
         for (auto e : pool) (void)hipEventDestroy(e);
         dfree(beads); dfree(sbeads); dfree(beads_alt); dfree(vel); dfree(acc); dfree(fpair); dfree(fscratch);
         dfree(k0); dfree(v0); dfree(k1); dfree(v1); dfree(hist[0]); dfree(hist[1]); dfree(cstart); dfree(cend);
+        dfree(k2); dfree(v2); dfree(wdesc);
         for (int p = 0; p < 2; p++) { dfree(kidx[p]); dfree(sk[p]); dfree(sv[p]); }
         dfree(mbefore); dfree(cnt_b); dfree(cnt_i); dfree(cnt_s); dfree(mflags); dfree(mcount);
         dfree(reps); dfree(glob); dfree(fconstF); dfree(uconstD); dfree(fconstD); dfree(chainPerBead); dfree(chainList);
@@ -384,13 +405,13 @@ struct sm_ctx {
     {
         uint32_t *gh = for_sort ? hist[sort_parity] : nullptr;
         // keys computed twice before a sort (sm_step_first called twice): start over
-        if (for_sort && hist_pending) launch_fill_u32(st, gh, sort_hist_words(), 0u);
+        if (for_sort && hist_pending) restart_hist(gh);
         hist_pending = hist_pending || for_sort;
         const int passes = sort_passes(sort_bits);
         timed(K_FIRST, [&] {
             if (for_sort)
                 launch_first(st, P, beads, vel, acc, reps, glob, key_out(), idx_out(), integrate, step_set, gh, passes,
-                             move_book());
+                             move_book(), win_keys());
             else   // keys only (no sort follows): into the mover buffer, which no sort state holds
                 launch_first(st, P, beads, vel, acc, reps, glob, k0, nullptr, integrate, step_set, nullptr, passes,
                              MoveBook{nullptr, nullptr});
@@ -399,7 +420,8 @@ struct sm_ctx {
     void sort_and_reorder()
     {
         const int p = sort_parity;
-        if (sort_mode) {
+        const bool win = win_now();
+        if (sort_mode == 1) {
             timed(K_SORT, [&] {
                 SortBuffers sb{k0, v0, k1, v1, {hist[0], hist[1]}, &glob->flags, n, 0};
                 const MoveSort m{kidx[kcur], order_valid ? kidx[1 - kcur] : nullptr, sk[scur], sv[scur], cstart, cend,
@@ -409,6 +431,12 @@ struct sm_ctx {
             scur = 1 - scur;
             kcur = 1 - kcur;
             order_valid = true;
+        } else if (win) {
+            timed(K_SORT, [&] {
+                SortBuffers sb{k0, v0, k1, v1, {hist[0], hist[1]}, &glob->flags, n, 0};
+                sort_window(st, sb, k2, v2, p, wdesc + 8 * p, reps);
+            });
+            order_valid = false;
         } else {
             timed(K_SORT, [&] {
                 SortBuffers sb{k0, v0, k1, v1, {hist[0], hist[1]}, &glob->flags, n, 0};
@@ -416,8 +444,13 @@ struct sm_ctx {
             });
             order_valid = false;
         }
+        win_last = win;
         timed(K_REORDER, [&] {
-            launch_reorder(st, n, skey(), sidx(), beads, sbeads, cstart, cend, hist[1 - p], hist_words);
+            if (win)
+                launch_reorder_win(st, n, k0, v0, beads, sbeads, cstart, cend, hist[1 - p], hist_words,
+                                   win_book(hist[p], n, wdesc + 8 * p), k2, wdesc + 8 * (1 - p), reps);
+            else
+                launch_reorder(st, n, skey(), sidx(), beads, sbeads, cstart, cend, hist[1 - p], hist_words);
         });
         sort_parity = 1 - p;
         hist_pending = false;
@@ -539,12 +572,12 @@ struct sm_ctx {
     {
         if (gexec[0] || gexec[1]) drop_graph();   // captured graphs hold the bead buffer
         uint32_t *gh = hist[sort_parity];
-        if (hist_pending) launch_fill_u32(st, gh, sort_hist_words(), 0u);
+        if (hist_pending) restart_hist(gh);
         hist_pending = true;
         const int passes = sort_passes(sort_bits);
         timed(K_KICK_FIRST, [&] {
             launch_kick_first(st, P, beads, beads_alt, chainPerBead, cconstD, gen(), fpair, vel, acc, reps, glob, s,
-                              key_out(), idx_out(), gh, passes, move_book());
+                              key_out(), idx_out(), gh, passes, move_book(), win_keys());
         });
         std::swap(beads, beads_alt);
     }
@@ -569,7 +602,7 @@ struct sm_ctx {
     // has one) falls on the block's last step.
     void graph_block(long long s)
     {
-        if (sort_mode && !order_valid) {
+        if (sort_mode == 1 && !order_valid) {
             // the graph would replay a first sort without a previous order: step this block
             // eagerly, after which the order is there to capture against
             for (int k = 0; k < 8; k++) eager_step(s + k);
@@ -895,14 +928,21 @@ sm_ctx *create(const sm_system *s, int nrep)
             c->sk[p] = dalloc<uint32_t>(n);
             c->sv[p] = dalloc<uint32_t>(n);
         }
-        c->sort_mode = n >= 1000000 ? 1 : 0;
+        c->win_ok = nrep == 1 && sortw_fits(n);
+        c->sort_mode = n >= 1000000 ? 1 : (c->win_ok ? 2 : 0);
+        if (c->win_ok) {
+            c->k2 = dalloc<uint32_t>(n);
+            c->v2 = dalloc<uint32_t>(n);
+            c->wdesc = dalloc<uint32_t>(16);
+            HIPCHK(hipMemsetAsync(c->wdesc, 0, sizeof(uint32_t) * 16, c->st));
+        }
         c->mbefore = dalloc<uint32_t>(n);
         c->cnt_b = dalloc<uint32_t>(move_blocks(n));
         c->cnt_i = dalloc<uint32_t>(move_tiles(n));
         c->cnt_s = dalloc<uint32_t>(move_tiles(n));
         c->mflags = dalloc<unsigned long long>(((size_t)n + 63) / 64);
         c->mcount = dalloc<int>(1);
-        c->hist_words = sort_scratch_words(n, 4);
+        c->hist_words = sort_scratch_words(n, 4) + (c->win_ok ? sortw_words(n) : 0);
         for (int p = 0; p < 2; p++) {
             c->hist[p] = dalloc<uint32_t>(c->hist_words);
             HIPCHK(hipMemsetAsync(c->hist[p], 0, sizeof(uint32_t) * c->hist_words, c->st));
@@ -1465,7 +1505,8 @@ int sm_tune(sm_ctx *c, const char *knob, int32_t value)
             if (value < 0 || value > 1) invalid("graph must be 0 or 1");
             c->use_graph = value != 0;
         } else if (std::strcmp(knob, "sort_mode") == 0) {
-            if (value < 0 || value > 1) invalid("sort_mode must be 0 or 1");
+            if (value < 0 || value > 2) invalid("sort_mode must be 0, 1 or 2");
+            if (value == 2 && !c->win_ok) invalid("sort_mode 2 needs a single replica of at most 524288 beads");
             c->sort_mode = value;
             c->order_valid = false;
         } else if (std::strcmp(knob, "mc_fused") == 0) {
@@ -1487,7 +1528,20 @@ int sm_tune_get(sm_ctx *c, const char *knob, int32_t *value)
         else if (std::strcmp(knob, "graph") == 0) *value = c->use_graph ? 1 : 0;
         else if (std::strcmp(knob, "sort_mode") == 0) *value = c->sort_mode;
         else if (std::strcmp(knob, "mc_fused") == 0) *value = c->mc_fused ? 1 : 0;
-        else invalid(std::string("unknown knob ") + knob);
+        else if (std::strcmp(knob, "sort_window_path") == 0) {
+            *value = 0;
+            if (c->win_last) {
+                // the last sort's window and fallback flag: its scratch is cleared only by the
+                // next sort's reorder
+                const int lp = 1 - c->sort_parity;
+                uint32_t wx = 0, flag = 0;
+                HIPCHK(hipMemcpyAsync(&wx, c->wdesc + 8 * lp + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+                HIPCHK(hipMemcpyAsync(&flag, win_book(c->hist[lp], c->n, nullptr).flag, sizeof(uint32_t),
+                                      hipMemcpyDeviceToHost, c->st));
+                HIPCHK(hipStreamSynchronize(c->st));
+                *value = (wx != 0 && flag == 0) ? 1 : 0;
+            }
+        } else invalid(std::string("unknown knob ") + knob);
     });
 }
 
@@ -1568,7 +1622,9 @@ int sm_autotune(sm_ctx *c, int32_t steps, double *best_ms_per_step)
                 }
             };
             for (const auto &k : cands) trial(k);
-            trial({bestc.chunk, bestc.variant, 1, 1 - bestc.sort});   // the other sort mode
+            const Cand b0 = bestc;
+            for (int m = 0; m <= 2; m++)   // the other sort modes
+                if (m != b0.sort && (m != 2 || c->win_ok)) trial({b0.chunk, b0.variant, 1, m});
             trial({bestc.chunk, bestc.variant, 0, bestc.sort});       // is the graph worth it here
             c->chunk = bestc.chunk;
             c->pair_variant = bestc.variant;

@@ -115,12 +115,41 @@ struct MoveBook {
     const uint32_t *kold;
     uint32_t *cnt_b;
 };
+// The window sort (sort_mode 2: one replica, every key sorted, keys of 17 to 24 bits, at
+// most kWinMaxTiles tiles).  Each sort has a window of cells: the previous sort's occupied
+// cell range widened by one cell on each side, used when it holds at most 2^18 cells.  A
+// key whose cell lies in it is stored as kWinTag | the cell's index in the window's own
+// (z, y, x) order, which orders keys as the cell index does, and two 9-bit passes sort it.
+// A key outside the window raises the sort's fallback flag; the same two launches then sort
+// the cell indices in three 8-bit passes.  desc == NULL: no window sort.
+constexpr uint32_t kWinTag = 1u << 31;
+constexpr int kWinMaxTiles = 256;
+struct WinBook {
+    const uint32_t *desc;   // this sort's window {x0, y0, z0, wx, wy, wz}; wx == 0: none
+    uint32_t *ghw;          // window digit histograms [16 copies][2 digits][512]
+    uint32_t *bbox;         // [16 copies][8]: max of cx + 1, cy + 1, cz + 1, 2^20 - cx, 2^20 - cy, 2^20 - cz
+    uint32_t *flag;         // nonzero: some key fell outside the window
+};
+// the window sort's scratch, after the sort_scratch_words(n, 4) words of the every-key sort:
+// ghw, bbox, {flag, 3 tile counters, 2 barrier counters, pad}, then 3 passes x tiles x 512
+// look-back words; the key kernel writes the first kWinKeyWords of it
+constexpr size_t kWinKeyWords = 16 * 1024 + 16 * 8 + 8;
+size_t sortw_words(int n);
+uint32_t *win_scratch(uint32_t *sort_scratch, int n);   // where it starts in a sort's scratch
+WinBook win_book(uint32_t *sort_scratch, int n, const uint32_t *desc);
+bool sortw_fits(int n);   // the tiles of n keys are few enough for the window sort's launches
 void launch_first(hipStream_t st, const Params &P, Bead *beads, float4 *vel, const float4 *acc, const DevRep *reps,
                   DevGlobal *glob, uint32_t *key, uint32_t *idx, int integrate, long long step_set, uint32_t *ghist,
-                  int passes, const MoveBook &book);
+                  int passes, const MoveBook &book, const WinBook &wb = WinBook{nullptr, nullptr, nullptr, nullptr});
 // also zeroes clear[0 .. clear_words) (the next sort's scratch)
 void launch_reorder(hipStream_t st, int n, const uint32_t *skey, const uint32_t *sidx, const Bead *beads,
                     Bead *sbeads, int *cstart, int *cend, uint32_t *clear, size_t clear_words);
+// The window sort's reorder: skey holds the sorted keys in either form, skey_full receives
+// the cell indices; it also derives the next sort's window (desc_next) from this sort's
+// occupied range.  The cell grid is replica 0's, read on the device.
+void launch_reorder_win(hipStream_t st, int n, const uint32_t *skey, const uint32_t *sidx, const Bead *beads,
+                        Bead *sbeads, int *cstart, int *cend, uint32_t *clear, size_t clear_words,
+                        const WinBook &wb, uint32_t *skey_full, uint32_t *desc_next, const DevRep *reps);
 // variant 0: one phase (FP64 test, force law under the branch) over per-quarter-cell culled
 // candidate lists; 1: FP32 screen into a per-lane hit list, then FP64 test + force law on
 // the hits
@@ -148,7 +177,8 @@ enum : int { KICK_STEP = 0, KICK_INIT = 1, KICK_NONE = 2 };
 void launch_kick_first(hipStream_t st, const Params &P, const Bead *beads, Bead *beads_out, const int2 *chain,
                        const double4 *cconst, const GenTables &gen, const float4 *fpair, float4 *vel, float4 *acc,
                        const DevRep *reps, DevGlobal *glob, long long step, uint32_t *key, uint32_t *idx,
-                       uint32_t *ghist, int passes, const MoveBook &book);
+                       uint32_t *ghist, int passes, const MoveBook &book,
+                       const WinBook &wb = WinBook{nullptr, nullptr, nullptr, nullptr});
 void launch_second(hipStream_t st, int n, const Bead *beads, const float4 *acc, float4 *vel, float dt);
 
 // FP64 gathers over the 27-cell stencil.  GATHER_U / GATHER_DU write one FP64 value per
@@ -236,5 +266,10 @@ struct MoveSort {
     uint32_t *skey, *sidx;                   // the new sorted order
 };
 void sort_moved(hipStream_t st, SortBuffers &b, const MoveSort &m, int bits, int parity);
+// The window sort of (k0,v0) (from a key kernel given win_book(scratch[parity], n, desc))
+// back into (k0,v0), through (k1,v1) and (k2,v2); the cell grid is replica 0's, read on the
+// device.  Needs sortw_fits(n) and scratch of sort_scratch_words(n, 4) + sortw_words(n) words.
+void sort_window(hipStream_t st, SortBuffers &b, uint32_t *k2, uint32_t *v2, int parity, const uint32_t *desc,
+                 const DevRep *reps);
 
 }  // namespace smdev

@@ -79,20 +79,27 @@ __device__ inline void drift_wrap(Bead &b, const float4 &v, double dt, double L0
 }
 
 // cell key of a bead (cellOpt.h:530-556), flagging a position outside the box
-__device__ inline uint32_t bead_key(const Bead &b, const DevRep &R, DevGlobal *__restrict__ glob)
+__device__ inline uint32_t bead_key(const Bead &b, const DevRep &R, DevGlobal *__restrict__ glob, int &cx, int &cy,
+                                    int &cz)
 {
     const int nx = R.nc[0], ny = R.nc[1], nz = R.nc[2];
     if (!(in_box(b.x, R.L[0]) && in_box(b.y, R.L[1]) && in_box(b.z, R.L[2]))) atomicOr(&glob->flags, FLAG_BAD_POSITION);
-    const int cx = cell_axis(b.x, R.cs[0], nx);
-    const int cy = cell_axis(b.y, R.cs[1], ny);
-    const int cz = cell_axis(b.z, R.cs[2], nz);
+    cx = cell_axis(b.x, R.cs[0], nx);
+    cy = cell_axis(b.y, R.cs[1], ny);
+    cz = cell_axis(b.z, R.cs[2], nz);
     return (uint32_t)(R.cellbase + (long long)(cx + cy * nx + cz * nx * ny));
+}
+__device__ inline uint32_t bead_key(const Bead &b, const DevRep &R, DevGlobal *__restrict__ glob)
+{
+    int cx, cy, cz;
+    return bead_key(b, R, glob, cx, cy, cz);
 }
 
 // Verlet::first of bead i and its cell key
 __device__ inline uint32_t first_one(int i, Bead *__restrict__ beads, float4 *__restrict__ vel,
                                      const float4 *__restrict__ acc, const DevRep *__restrict__ reps,
-                                     DevGlobal *__restrict__ glob, float halfdt, double dt, int integrate)
+                                     DevGlobal *__restrict__ glob, float halfdt, double dt, int integrate, int &cx,
+                                     int &cy, int &cz)
 {
     Bead b = beads[i];
     const DevRep &R = reps[b.rep];
@@ -103,7 +110,7 @@ __device__ inline uint32_t first_one(int i, Bead *__restrict__ beads, float4 *__
         vel[i] = v;
         beads[i] = b;
     }
-    return bead_key(b, R, glob);
+    return bead_key(b, R, glob, cx, cy, cz);
 }
 
 // The incremental re-sort's mover test for bead i with key k (see MoveBook); returns
@@ -125,18 +132,28 @@ __global__ __launch_bounds__(kBlock) void k_first(int n, Bead *__restrict__ bead
                                                   DevGlobal *__restrict__ glob, uint32_t *__restrict__ key,
                                                   uint32_t *__restrict__ idx, float halfdt, double dt, int integrate,
                                                   long long step_set, uint32_t *__restrict__ ghist, int passes,
-                                                  MoveBook book)
+                                                  MoveBook book, WinBook wb)
 {
-    __shared__ uint32_t h[4][256];
+    __shared__ uint32_t hs[1024];
+    uint32_t (*h)[256] = reinterpret_cast<uint32_t (*)[256]>(hs);
     const int i = blockIdx.x * kBlock + threadIdx.x;
+    const bool win = wb.desc != nullptr;
+    WinDesc w{};
+    if (win) w = load_win(wb.desc);
     if (i == 0 && step_set >= -1) glob->step = step_set >= 0 ? step_set : glob->step + 1;   // no other reader here
     if (ghist) {
-        for (int p = 0; p < passes; p++) h[p][threadIdx.x] = 0u;
+        if (win)
+            for (int j = 0; j < 4; j++) hs[threadIdx.x + 256 * j] = 0u;
+        else
+            for (int p = 0; p < passes; p++) h[p][threadIdx.x] = 0u;
         __syncthreads();
     }
     uint32_t k = 0u;
-    if (i < n) k = first_one(i, beads, vel, acc, reps, glob, halfdt, dt, integrate);
-    if (ghist) {
+    int cx = 0, cy = 0, cz = 0;
+    if (i < n) k = first_one(i, beads, vel, acc, reps, glob, halfdt, dt, integrate, cx, cy, cz);
+    if (ghist && win) {
+        k = win_key_end(wb, w, hs, k, cx, cy, cz, i < n);
+    } else if (ghist) {
         // the sort's digit histograms while the key is in a register (all lanes: ballots)
         const bool counted = note_move(book, i, k, i < n);
         hist_add(h, k, counted, passes);
@@ -168,6 +185,68 @@ __global__ __launch_bounds__(kBlock) void k_reorder(int n, const uint32_t *__res
     sbeads[s] = beads[sidx[s]];
     if (s == 0 || kprev != k) cstart[k] = s;
     if (s == n - 1 || knext != k) cend[k] = s + 1;
+}
+
+// k_reorder after a window sort: the sorted keys are window keys (the window path) or cell
+// indices (the fallback); either way equal keys are equal cells.  The cell indices go to
+// skf for the pair kernels.  Block 0 also sets the next sort's window: this sort's after
+// the window path; after the fallback, the occupied cell range the fallback measured,
+// widened by one cell on each side and clipped to the grid, or none (wx = 0) when that
+// holds more than 2^18 cells.
+__global__ __launch_bounds__(kBlock) void k_reorder_win(int n, const uint32_t *__restrict__ skey,
+                                                        const uint32_t *__restrict__ sidx,
+                                                        const Bead *__restrict__ beads, Bead *__restrict__ sbeads,
+                                                        int *__restrict__ cstart, int *__restrict__ cend,
+                                                        uint32_t *__restrict__ clear, size_t clear_words, WinBook wb,
+                                                        uint32_t *__restrict__ skf, uint32_t *__restrict__ desc_next,
+                                                        const DevRep *__restrict__ reps)
+{
+    const int nx = reps[0].nc[0], ny = reps[0].nc[1], nz = reps[0].nc[2];
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    for (size_t w = (size_t)s; w < clear_words; w += (size_t)gridDim.x * kBlock) clear[w] = 0u;
+    const WinDesc w = load_win(wb.desc);
+    const bool fallback = w.wx == 0u || *wb.flag != 0u;
+    if (blockIdx.x == 0 && threadIdx.x < kWave) {
+        const int t = threadIdx.x;
+        uint32_t v = 0u;
+        if (fallback && t < 6)
+            for (int c = 0; c < kHistCopies; c++) v = max(v, wb.bbox[c * 8 + t]);
+        uint32_t f[6];
+#pragma unroll
+        for (int j = 0; j < 6; j++) f[j] = (uint32_t)__shfl((int)v, j);
+        if (t == 0) {
+            uint32_t d[6] = {w.x0, w.y0, w.z0, w.wx, w.wy, w.wz};
+            if (fallback) {
+                for (int j = 0; j < 6; j++) d[j] = 0u;
+                if (f[0] != 0u) {
+                    const int ncell[3] = {nx, ny, nz};
+                    unsigned long long cells = 1ull;
+                    for (int a = 0; a < 3; a++) {
+                        const int lo = max((int)(kBoxEnc - f[3 + a]) - 1, 0);
+                        const int hi = min((int)f[a], ncell[a] - 1);   // f = highest cell + 1
+                        d[a] = (uint32_t)lo;
+                        d[3 + a] = (uint32_t)(hi - lo + 1);
+                        cells *= (unsigned long long)(hi - lo + 1);
+                    }
+                    if (cells > (1ull << 18))
+                        for (int j = 0; j < 6; j++) d[j] = 0u;
+                }
+            }
+            for (int j = 0; j < 6; j++) desc_next[j] = d[j];
+        }
+    }
+    if (s >= n) return;
+    const uint32_t k = skey[s], kprev = skey[s > 0 ? s - 1 : 0], knext = skey[s < n - 1 ? s + 1 : s];
+    sbeads[s] = beads[sidx[s]];
+    uint32_t full = k;
+    if (!fallback) {
+        const uint32_t wk = k & ~kWinTag;
+        const uint32_t q = wk / w.wx, x = wk - q * w.wx, z = q / w.wy, y = q - z * w.wy;
+        full = ((z + w.z0) * (uint32_t)ny + (y + w.y0)) * (uint32_t)nx + (x + w.x0);
+    }
+    skf[s] = full;
+    if (s == 0 || kprev != k) cstart[full] = s;
+    if (s == n - 1 || knext != k) cend[full] = s + 1;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -347,7 +426,8 @@ __global__ __launch_bounds__(kBlock) void k_bonded_kick(int n, const Bead *__res
 // argument (no thread reads the device counter, which thread 0 advances to step + 1).
 // ONE: a single-replica context, whose replica constants are read from reps[0] without
 // waiting for the bead's replica index
-template <bool ONE>
+// WIN: keys for a window sort (wb)
+template <bool ONE, bool WIN>
 __global__ __launch_bounds__(kBlock) void k_kick_first(int n, const Bead *__restrict__ beads,
                                                        Bead *__restrict__ beads_out,
                                                        const int2 *__restrict__ chain,
@@ -357,14 +437,21 @@ __global__ __launch_bounds__(kBlock) void k_kick_first(int n, const Bead *__rest
                                                        DevGlobal *__restrict__ glob, float halfdt, double dt,
                                                        long long step, uint32_t *__restrict__ key,
                                                        uint32_t *__restrict__ idx, uint32_t *__restrict__ ghist,
-                                                       int passes, MoveBook book)
+                                                       int passes, MoveBook book, WinBook wb)
 {
-    __shared__ uint32_t h[4][256];
+    __shared__ uint32_t hs[1024];
+    uint32_t (*h)[256] = reinterpret_cast<uint32_t (*)[256]>(hs);
     const int i = blockIdx.x * kBlock + threadIdx.x;
+    WinDesc w{};
+    if (WIN) w = load_win(wb.desc);
     if (i == 0) glob->step = step + 1;
-    for (int p = 0; p < passes; p++) h[p][threadIdx.x] = 0u;
+    if (WIN)
+        for (int j = 0; j < 4; j++) hs[threadIdx.x + 256 * j] = 0u;
+    else
+        for (int p = 0; p < passes; p++) h[p][threadIdx.x] = 0u;
     __syncthreads();
     uint32_t k = 0u;
+    int cx = 0, cy = 0, cz = 0;
     if (i < n) {
         Bead b = beads[i];
         const DevRep &R = reps[ONE ? 0 : b.rep];
@@ -379,13 +466,17 @@ __global__ __launch_bounds__(kBlock) void k_kick_first(int n, const Bead *__rest
             vel[i] = v;
         }
         beads_out[i] = b;
-        k = bead_key(b, R, glob);
+        k = bead_key(b, R, glob, cx, cy, cz);
     }
-    const bool counted = note_move(book, i, k, i < n);
-    hist_add(h, k, counted, passes);
-    const int nmoved = __syncthreads_count(counted);
-    hist_flush(h, ghist, passes);
-    flush_moves(book, nmoved);
+    if (WIN) {
+        k = win_key_end(wb, w, hs, k, cx, cy, cz, i < n);
+    } else {
+        const bool counted = note_move(book, i, k, i < n);
+        hist_add(h, k, counted, passes);
+        const int nmoved = __syncthreads_count(counted);
+        hist_flush(h, ghist, passes);
+        flush_moves(book, nmoved);
+    }
     if (i < n) {
         key[i] = k;
         if (idx) idx[i] = (uint32_t)i;
@@ -763,10 +854,10 @@ inline int blocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 // ---------------------------------------------------------------------------------------
 void launch_first(hipStream_t st, const Params &P, Bead *beads, float4 *vel, const float4 *acc, const DevRep *reps,
                   DevGlobal *glob, uint32_t *key, uint32_t *idx, int integrate, long long step_set, uint32_t *ghist,
-                  int passes, const MoveBook &book)
+                  int passes, const MoveBook &book, const WinBook &wb)
 {
     hipLaunchKernelGGL(k_first, dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, beads, vel, acc, reps, glob, key, idx,
-                       (float)(0.5 * P.dt), P.dt, integrate, step_set, ghist, passes, book);
+                       (float)(0.5 * P.dt), P.dt, integrate, step_set, ghist, passes, book, wb);
 }
 
 void launch_reorder(hipStream_t st, int n, const uint32_t *skey, const uint32_t *sidx, const Bead *beads,
@@ -774,6 +865,14 @@ void launch_reorder(hipStream_t st, int n, const uint32_t *skey, const uint32_t 
 {
     hipLaunchKernelGGL(k_reorder, dim3(blocks(n)), dim3(kBlock), 0, st, n, skey, sidx, beads, sbeads, cstart, cend,
                        clear, clear_words);
+}
+
+void launch_reorder_win(hipStream_t st, int n, const uint32_t *skey, const uint32_t *sidx, const Bead *beads,
+                        Bead *sbeads, int *cstart, int *cend, uint32_t *clear, size_t clear_words,
+                        const WinBook &wb, uint32_t *skey_full, uint32_t *desc_next, const DevRep *reps)
+{
+    hipLaunchKernelGGL(k_reorder_win, dim3(blocks(n)), dim3(kBlock), 0, st, n, skey, sidx, beads, sbeads, cstart,
+                       cend, clear, clear_words, wb, skey_full, desc_next, reps);
 }
 
 void launch_bonded_kick(hipStream_t st, const Params &P, const Bead *beads, const int2 *chain,
@@ -787,17 +886,21 @@ void launch_bonded_kick(hipStream_t st, const Params &P, const Bead *beads, cons
 void launch_kick_first(hipStream_t st, const Params &P, const Bead *beads, Bead *beads_out, const int2 *chain,
                        const double4 *cconst, const GenTables &gen, const float4 *fpair, float4 *vel, float4 *acc,
                        const DevRep *reps, DevGlobal *glob, long long step, uint32_t *key, uint32_t *idx,
-                       uint32_t *ghist, int passes, const MoveBook &book)
+                       uint32_t *ghist, int passes, const MoveBook &book, const WinBook &wb)
 {
     const float halfdt = (float)(P.dt * 0.5);
-    if (P.nrep == 1)
-        hipLaunchKernelGGL((k_kick_first<true>), dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, beads, beads_out,
+    if (P.nrep == 1 && wb.desc)
+        hipLaunchKernelGGL((k_kick_first<true, true>), dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, beads, beads_out,
                            chain, cconst, gen, fpair, vel, acc, reps, glob, halfdt, P.dt, step, key, idx, ghist,
-                           passes, book);
+                           passes, book, wb);
+    else if (P.nrep == 1)
+        hipLaunchKernelGGL((k_kick_first<true, false>), dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, beads,
+                           beads_out, chain, cconst, gen, fpair, vel, acc, reps, glob, halfdt, P.dt, step, key, idx,
+                           ghist, passes, book, wb);
     else
-        hipLaunchKernelGGL((k_kick_first<false>), dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, beads, beads_out,
-                           chain, cconst, gen, fpair, vel, acc, reps, glob, halfdt, P.dt, step, key, idx, ghist,
-                           passes, book);
+        hipLaunchKernelGGL((k_kick_first<false, false>), dim3(blocks(P.n)), dim3(kBlock), 0, st, P.n, beads,
+                           beads_out, chain, cconst, gen, fpair, vel, acc, reps, glob, halfdt, P.dt, step, key, idx,
+                           ghist, passes, book, wb);
 }
 
 void launch_second(hipStream_t st, int n, const Bead *beads, const float4 *acc, float4 *vel, float halfdt)

@@ -51,4 +51,45 @@ __device__ inline void hist_flush(uint32_t (*h)[256], uint32_t *ghist, int passe
         if (h[p][threadIdx.x]) atomicAdd(&g[p * 256 + threadIdx.x], h[p][threadIdx.x]);
 }
 
+// ---- the window sort: the key kernel's side ----------------------------------------------
+struct WinDesc {
+    uint32_t x0, y0, z0, wx, wy, wz;
+};
+__device__ inline WinDesc load_win(const uint32_t *d) { return WinDesc{d[0], d[1], d[2], d[3], d[4], d[5]}; }
+
+// occupied-range words hold max(c + 1) and max(kBoxEnc - c) per axis, so that zero is empty
+constexpr uint32_t kBoxEnc = 1u << 20;
+
+// The end of a key kernel's block in a window sort; every thread of the 256-thread block
+// calls it and gets the key to store.  A key in the window is counted by its two window
+// digits (bits 0-8 and 9-17); a key outside it sets the fallback flag.  With no window the
+// key is stored as it is (the fallback counts the digits).  hs: the block's 1024 LDS
+// words, zero on entry.
+__device__ inline uint32_t win_key_end(const WinBook &wb, const WinDesc &w, uint32_t *hs, uint32_t k, int cx, int cy,
+                                       int cz, bool valid)
+{
+    const int t = threadIdx.x, lane = t & (kWave - 1);
+    if (w.wx == 0u) return k;   // workgroup-uniform
+    bool inw = false;
+    uint32_t wk = 0u;
+    if (valid) {
+        const uint32_t dx = (uint32_t)cx - w.x0, dy = (uint32_t)cy - w.y0, dz = (uint32_t)cz - w.z0;
+        inw = dx < w.wx && dy < w.wy && dz < w.wz;
+        wk = (dz * w.wy + dy) * w.wx + dx;
+    }
+    if (inw) {
+        atomicAdd(&hs[wk & 511u], 1u);
+        atomicAdd(&hs[512u + ((wk >> 9) & 511u)], 1u);
+    }
+    if (__ballot(valid && !inw) != 0ull && lane == 0) atomicOr(wb.flag, 1u);
+    __syncthreads();
+    uint32_t *g = wb.ghw + (size_t)(blockIdx.x % kHistCopies) * 1024;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t c = hs[t + 256 * j];
+        if (c) atomicAdd(&g[t + 256 * j], c);
+    }
+    return inw ? (kWinTag | wk) : k;
+}
+
 }  // namespace smdev

@@ -306,7 +306,7 @@ def test_fused_box_move_dU_keeps_the_trajectory(name):
 
 
 
-@pytest.mark.parametrize("sort_mode", [0, 1])
+@pytest.mark.parametrize("sort_mode", [0, 1, 2])
 def test_systems_smaller_than_a_wave(sort_mode):
     """One bead, two beads, and one 3-bead lipid chain beside a free bead, in a 10^3 box:
     fewer beads than a wave, a key sort of one partial tile, and pair-kernel chunks past
