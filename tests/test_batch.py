@@ -176,9 +176,9 @@ def test_knobs_validate_and_round_trip():
     if not gpu_available():
         pytest.skip("no GPU")
     s = sm.Solver.from_system(sm.Mpd.bilayer(1234, 250, 3.45).system())
-    valid = {"pair_chunk": (1, 8, 4096), "pair_variant": (0, 1), "graph": (0, 1), "sort_mode": (0, 1),
+    valid = {"pair_chunk": (1, 8, 4096), "pair_variant": (0, 1), "graph": (0, 1), "sort_mode": (0, 1, 2),
              "mc_fused": (0, 1)}
-    invalid = {"pair_chunk": (0, -1, 4097), "pair_variant": (-1, 2), "graph": (2,), "sort_mode": (2,),
+    invalid = {"pair_chunk": (0, -1, 4097), "pair_variant": (-1, 2), "graph": (2,), "sort_mode": (-1, 3),
                "mc_fused": (-1, 2)}
     for knob, values in valid.items():
         for v in values:
