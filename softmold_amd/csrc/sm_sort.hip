@@ -461,7 +461,12 @@ constexpr int kWDig = 512;
 
 template <int kRounds>
 struct WLds {
-    uint16_t wc[kRounds][kWavesPerBlock][kWDig];   // per (round, wave, digit): count, then tile-local offset
+    // per (round, wave, digit): count, then tile-local offset; a thread owning two adjacent
+    // digits reads and writes both as one word
+    union {
+        uint16_t h[kRounds][kWavesPerBlock][kWDig];
+        uint32_t w[kRounds][kWavesPerBlock][kWDig / 2];
+    } wc;
     uint32_t dpos[kWDig];                          // per digit: output position of the tile's first item
     uint32_t h[3][256];                            // fallback: digit counts of a tile's cell indices
     uint32_t bb[kWavesPerBlock][6];                // fallback: occupied cell range of each wave's keys
@@ -611,9 +616,12 @@ __device__ void wpass(WLds<kRounds> &L, int n, int shift, int tile, const uint32
 #pragma unroll
     for (int r = 0; r < kRounds; r++)
 #pragma unroll
-        for (int w = 0; w < kWavesPerBlock; w++)
-#pragma unroll
-            for (int j = 0; j < kPer; j++) L.wc[r][w][t * kPer + j] = 0;
+        for (int w = 0; w < kWavesPerBlock; w++) {
+            if (kPer == 2)
+                L.wc.w[r][w][t] = 0u;
+            else
+                L.wc.h[r][w][t] = 0;
+        }
     __syncthreads();
     uint32_t wpre = 0;
     for (int w = 0; w < wave; w++) wpre += L.scan[w];
@@ -628,33 +636,40 @@ __device__ void wpass(WLds<kRounds> &L, int n, int shift, int tile, const uint32
         dr[r] = valid ? (kr[r] >> shift) & (uint32_t)(kDig - 1) : (uint32_t)kDig;   // kDig: no item
         const unsigned long long peers = match_bits<kBits>(dr[r] & (uint32_t)(kDig - 1), valid);
         rk[r] = (uint32_t)__popcll(peers & lt);
-        if (valid && rk[r] == 0u) L.wc[r][wave][dr[r]] = (uint16_t)__popcll(peers);
+        if (valid && rk[r] == 0u) L.wc.h[r][wave][dr[r]] = (uint16_t)__popcll(peers);
     }
     __syncthreads();
 
     // per digit: the counts read together, tile-local offsets written, the tile's count
     // published for later tiles
     uint32_t mine[kPer];
-#pragma unroll
-    for (int j = 0; j < kPer; j++) {
-        const int d = t * kPer + j;
+    {
         uint32_t cnt[kRounds][kWavesPerBlock];
 #pragma unroll
         for (int r = 0; r < kRounds; r++)
 #pragma unroll
-            for (int w = 0; w < kWavesPerBlock; w++) cnt[r][w] = L.wc[r][w][d];
-        uint32_t off = 0;
+            for (int w = 0; w < kWavesPerBlock; w++) cnt[r][w] = kPer == 2 ? L.wc.w[r][w][t] : L.wc.h[r][w][t];
+        uint32_t off0 = 0, off1 = 0;
 #pragma unroll
         for (int r = 0; r < kRounds; r++)
 #pragma unroll
             for (int w = 0; w < kWavesPerBlock; w++) {
-                L.wc[r][w][d] = (uint16_t)off;
-                off += cnt[r][w];
+                if (kPer == 2) {
+                    L.wc.w[r][w][t] = off0 | (off1 << 16);
+                    off0 += cnt[r][w] & 0xFFFFu;
+                    off1 += cnt[r][w] >> 16;
+                } else {
+                    L.wc.h[r][w][t] = (uint16_t)off0;
+                    off0 += cnt[r][w];
+                }
             }
-        mine[j] = off;
-        __hip_atomic_store(status + (size_t)tile * kWDig + d, (tile == 0 ? kFlagIncl : kFlagAgg) | off,
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        mine[0] = off0;
+        if (kPer == 2) mine[kPer - 1] = off1;
     }
+#pragma unroll
+    for (int j = 0; j < kPer; j++)
+        __hip_atomic_store(status + (size_t)tile * kWDig + t * kPer + j, (tile == 0 ? kFlagIncl : kFlagAgg) | mine[j],
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t prefix[kPer];
     if (kPer == 2)
         look_back2(status, tile, t * kPer, t * kPer + kPer - 1, prefix[0], prefix[kPer - 1], errflag);
@@ -673,7 +688,7 @@ __device__ void wpass(WLds<kRounds> &L, int n, int shift, int tile, const uint32
 #pragma unroll
     for (int r = 0; r < kRounds; r++) {
         if (dr[r] < (uint32_t)kDig) {
-            uint32_t p = L.dpos[dr[r]] + L.wc[r][wave][dr[r]] + rk[r];
+            uint32_t p = L.dpos[dr[r]] + L.wc.h[r][wave][dr[r]] + rk[r];
             p = p < (uint32_t)n ? p : (uint32_t)(n - 1);   // only reachable after a timeout
             kout[p] = kr[r];
             vout[p] = vr[r];
