@@ -227,7 +227,8 @@ struct sm_ctx {
     const uint32_t *sidx() const { return sort_mode == 1 ? sv[scur] : (win_last ? v0 : (sorted_sel ? v1 : v0)); }
     // the key kernel's outputs and bookkeeping for the next sort
     uint32_t *key_out() const { return sort_mode == 1 ? kidx[kcur] : k0; }
-    uint32_t *idx_out() const { return sort_mode == 1 ? nullptr : v0; }
+    // the window sort's first pass takes the bead indices from the keys' positions
+    uint32_t *idx_out() const { return sort_mode == 1 || win_now() ? nullptr : v0; }
     WinBook win_keys() const
     {
         if (!win_now()) return WinBook{nullptr, nullptr, nullptr, nullptr};

@@ -266,8 +266,9 @@ struct MoveSort {
     uint32_t *skey, *sidx;                   // the new sorted order
 };
 void sort_moved(hipStream_t st, SortBuffers &b, const MoveSort &m, int bits, int parity);
-// The window sort of (k0,v0) (from a key kernel given win_book(scratch[parity], n, desc))
-// back into (k0,v0), through (k1,v1) and (k2,v2); the cell grid is replica 0's, read on the
+// The window sort of the keys k0 by bead index (from a key kernel given
+// win_book(scratch[parity], n, desc); v0 is not read) into (k0,v0), through (k1,v1) and
+// (k2,v2); the cell grid is replica 0's, read on the
 // device.  Needs sortw_fits(n) and scratch of sort_scratch_words(n, 4) + sortw_words(n) words.
 void sort_window(hipStream_t st, SortBuffers &b, uint32_t *k2, uint32_t *v2, int parity, const uint32_t *desc,
                  const DevRep *reps);

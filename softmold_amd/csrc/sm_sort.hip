@@ -586,8 +586,9 @@ __device__ inline void look_back2(const uint32_t *status, int tile, int d0, int 
 
 // One stable pass over the kBits-bit digit at `shift` of tile `tile`, whose items take
 // digit bases from the per-thread histogram sums g (the thread's kPer digits, over every
-// copy).
-template <int kRounds, int kBits>
+// copy).  kIota: the first pass, whose values are the keys' positions (bead indices), not
+// loaded from vin.
+template <int kRounds, int kBits, bool kIota = false>
 __device__ void wpass(WLds<kRounds> &L, int n, int shift, int tile, const uint32_t (&g)[(1 << kBits) / kThreads],
                       const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin, uint32_t *__restrict__ kout,
                       uint32_t *__restrict__ vout, uint32_t *__restrict__ status, int *__restrict__ errflag)
@@ -601,7 +602,7 @@ __device__ void wpass(WLds<kRounds> &L, int n, int shift, int tile, const uint32
     for (int r = 0; r < kRounds; r++) {
         const int i = min(base + r * kThreads + t, n - 1);
         kr[r] = kin[i];
-        vr[r] = vin[i];
+        vr[r] = kIota ? (uint32_t)i : vin[i];
     }
     uint32_t gs = 0u;
 #pragma unroll
@@ -777,7 +778,7 @@ __global__ __launch_bounds__(kThreads) void k_sortw(int n, int launch, WArgs a)
     const int tile = (int)L.tile;
     if (!fallback) {
         if (launch == 0) {
-            wpass<kRounds, 9>(L, n, 0, tile, gw, a.kA, a.vA, a.kB, a.vB, st0, a.errflag);
+            wpass<kRounds, 9, true>(L, n, 0, tile, gw, a.kA, a.vA, a.kB, a.vB, st0, a.errflag);
         } else if (w.wx * w.wy * w.wz <= (1u << 17)) {   // window keys of 17 bits: an 8-bit second digit
             wpass<kRounds, 8>(L, n, 9, tile, gw8, a.kB, a.vB, a.kA, a.vA, st1, a.errflag);
         } else {
@@ -836,7 +837,7 @@ __global__ __launch_bounds__(kThreads) void k_sortw(int n, int launch, WArgs a)
     }
     grid_sync(a.bar, gridDim.x, a.errflag);
     hist_sums<1>(a.ghist, g8);
-    wpass<kRounds, 8>(L, n, 0, tile, g8, a.kA, a.vA, a.kB, a.vB, st0, a.errflag);
+    wpass<kRounds, 8, true>(L, n, 0, tile, g8, a.kA, a.vA, a.kB, a.vB, st0, a.errflag);
     grid_sync(a.bar + 1, gridDim.x, a.errflag);
     const uint32_t tile1 = draw_tile(a.ctr + 2);
     hist_sums<1>(a.ghist + 256, g8);
