@@ -25,7 +25,8 @@ def system(name):
     if name not in _systems:
         m = {"L500": lambda: sm.Mpd.liposome(1234, 500, 3.45),
              "L5000": lambda: sm.Mpd.liposome(1234, 5000, 3.45),
-             "L80000": lambda: sm.Mpd.liposome(1234, 80000, 3.45)}[name]()
+             "L80000": lambda: sm.Mpd.liposome(1234, 80000, 3.45),
+             "L100000": lambda: sm.Mpd.liposome(1234, 100000, 3.45)}[name]()
         _systems[name] = m.system()
         m.close()
     return _systems[name]
@@ -48,6 +49,7 @@ def _same(a, b):
     ("L500", {}, 200),                                           # 512-key tiles
     ("L5000", {}, 400),
     ("L80000", {}, 200),                                         # 2048-key tiles, the bench's system
+    ("L100000", {}, 100),                                        # a window of over 2^17 cells: 9 + 9 bits
     ("L5000", dict(delta_lxy=0.05, tension=1.0, seed=5), 400),   # box moves every 8 steps
 ])
 def test_window_sort_equals_full_sort(name, kw, steps):
